@@ -1,0 +1,901 @@
+// cr_runtime.hip -- host runtime and C ABI of the gfx950 byte-pattern replace library.
+//
+// A session keeps the buffer resident in HBM as a ping-pong pair (d_buf holds the
+// current contents, d_tmp receives the next pass) plus the scratch the single-pass
+// kernel needs: look-back state words, per-tile records, a ticket counter and a
+// small stats block.  Every apply is: upload pattern + replacement (pinned staging,
+// async), one kernel launch (two for long self-overlapping patterns, three for the
+// bitmap fallback), one 32-byte stats readback, swap.
+//
+// Mirrors the reference's session model (cuda_replace.cu:139-216 CRContext,
+// 997-1140 ensure_* allocators, 2097-2159 apply_once, 2161-2603 exports) with
+// 64-bit lengths and no per-call device allocation of pattern buffers.
+#include "cr_internal.h"
+#include "../../include/cuda_replace.h"
+
+#include <limits.h>
+#include <mutex>
+#include <new>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+using namespace cr;
+
+namespace {
+
+constexpr size_t kAllocGranule = 2u << 20;   // device buffers grow in 2 MiB steps
+constexpr int kModeEmpty = 4;                // empty pattern (bytes.replace inserts)
+
+struct Session {
+    std::mutex mtx;
+    bool alive = true;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    uint8_t *d_buf = nullptr;
+    size_t cap_buf = 0;
+    uint8_t *d_tmp = nullptr;
+    size_t cap_tmp = 0;
+    size_t len = 0;
+
+    uint64_t *d_state = nullptr;
+    size_t state_words = 0;
+    TileInfo *d_info = nullptr;
+    size_t info_tiles = 0;
+    uint32_t *d_bitmap = nullptr;
+    size_t bitmap_words = 0;
+    uint8_t *d_patrep = nullptr;
+    size_t patrep_cap = 0;
+    uint8_t *h_patrep = nullptr;     // pinned staging for pattern + replacement
+    size_t h_patrep_cap = 0;
+    uint64_t *d_presence = nullptr;
+    size_t presence_words = 0;
+
+    // small device block: ticket counter, stats, query output
+    uint8_t *d_misc = nullptr;
+    uint8_t *h_misc = nullptr;       // pinned mirror
+    unsigned long long *d_ticket = nullptr;
+    LaunchStats *d_stats = nullptr;
+    int64_t *d_query = nullptr;
+    uint64_t ticket_base = 0;
+    uint32_t epoch = 0;
+
+    // record of the most recent apply, for prefix queries
+    bool q_valid = false;
+    int q_mode = 0;
+    ReplaceParams q_params{};
+};
+
+struct DeviceGuard {
+    int prev = -1;
+    bool changed = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
+            changed = hipSetDevice(dev) == hipSuccess;
+        }
+    }
+    ~DeviceGuard()
+    {
+        if (changed)
+            (void)hipSetDevice(prev);
+    }
+};
+
+#define CK(expr)                                  \
+    do {                                          \
+        hipError_t e_ = (expr);                   \
+        if (e_ != hipSuccess)                     \
+            return (int)e_;                       \
+    } while (0)
+
+size_t round_up(size_t v, size_t g) { return (v + g - 1) / g * g; }
+
+// grow a device allocation without preserving contents
+hipError_t grow(void **ptr, size_t *cap, size_t need_bytes, bool zero, hipStream_t s)
+{
+    if (*cap >= need_bytes && *ptr)
+        return hipSuccess;
+    size_t nc = round_up(need_bytes > 0 ? need_bytes : 1, kAllocGranule);
+    if (*cap > 0 && nc < *cap + *cap / 2)
+        nc = round_up(*cap + *cap / 2, kAllocGranule);
+    void *np = nullptr;
+    hipError_t e = hipMalloc(&np, nc);
+    if (e != hipSuccess) {
+        // retry with the exact size before giving up
+        (void)hipGetLastError();
+        nc = round_up(need_bytes > 0 ? need_bytes : 1, kAllocGranule);
+        e = hipMalloc(&np, nc);
+        if (e != hipSuccess)
+            return e;
+    }
+    if (zero) {
+        e = hipMemsetAsync(np, 0, nc, s);
+        if (e != hipSuccess) {
+            (void)hipFree(np);
+            return e;
+        }
+    }
+    if (*ptr) {
+        // the old buffer may still be read by queued work on this stream
+        hipError_t se = hipStreamSynchronize(s);
+        (void)hipFree(*ptr);
+        if (se != hipSuccess) {
+            *ptr = np;
+            *cap = nc;
+            return se;
+        }
+    }
+    *ptr = np;
+    *cap = nc;
+    return hipSuccess;
+}
+
+hipError_t grow_host(uint8_t **ptr, size_t *cap, size_t need)
+{
+    if (*cap >= need && *ptr)
+        return hipSuccess;
+    size_t nc = round_up(need > 0 ? need : 1, 4096);
+    void *np = nullptr;
+    hipError_t e = hipHostMalloc(&np, nc, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return e;
+    if (*ptr)
+        (void)hipHostFree(*ptr);
+    *ptr = (uint8_t *)np;
+    *cap = nc;
+    return hipSuccess;
+}
+
+// Pattern is border-free iff its longest proper border (KMP failure value) is 0;
+// then two occurrences can never overlap.
+bool border_free(const uint8_t *pat, int L)
+{
+    if (L <= 1)
+        return true;
+    std::vector<int> f((size_t)L, 0);
+    int k = 0;
+    for (int i = 1; i < L; ++i) {
+        while (k > 0 && pat[i] != pat[k])
+            k = f[(size_t)k - 1];
+        if (pat[i] == pat[k])
+            ++k;
+        f[(size_t)i] = k;
+    }
+    return f[(size_t)L - 1] == 0;
+}
+
+int pattern_mode(const uint8_t *pat, int L, int *table_size)
+{
+    if (L == 0)
+        return kModeEmpty;
+    const bool bf = border_free(pat, L);
+    if (bf && L <= kMaxSinglePassPat)
+        return kModeBorderFree;
+    if (!bf && L <= kMaxTablePat) {
+        int lt = 2;
+        while (lt < L)
+            lt <<= 1;
+        if (table_size)
+            *table_size = lt;
+        return kModeTable;
+    }
+    return kModeBitmap;
+}
+
+hipError_t ensure_misc(Session *s)
+{
+    if (s->d_misc)
+        return hipSuccess;
+    const size_t bytes = 256;
+    hipError_t e = hipMalloc((void **)&s->d_misc, bytes);
+    if (e != hipSuccess)
+        return e;
+    e = hipMemsetAsync(s->d_misc, 0, bytes, s->stream);
+    if (e != hipSuccess)
+        return e;
+    e = hipHostMalloc((void **)&s->h_misc, bytes, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return e;
+    s->d_ticket = (unsigned long long *)s->d_misc;
+    s->d_stats = (LaunchStats *)(s->d_misc + 64);
+    s->d_query = (int64_t *)(s->d_misc + 128);
+    s->ticket_base = 0;
+    return hipStreamSynchronize(s->stream);
+}
+
+// next epoch; clear the state words when the 16-bit epoch wraps
+hipError_t next_epoch(Session *s)
+{
+    if (s->epoch >= 0xffffu) {
+        if (s->d_state) {
+            hipError_t e = launch_fill_u64(s->d_state, 0, s->state_words, s->stream);
+            if (e != hipSuccess)
+                return e;
+        }
+        hipError_t e = hipMemsetAsync(s->d_stats, 0, sizeof(LaunchStats), s->stream);
+        if (e != hipSuccess)
+            return e;
+        s->epoch = 0;
+    }
+    ++s->epoch;
+    return hipSuccess;
+}
+
+hipError_t ensure_scratch(Session *s, int64_t num_tiles, int mode, int table_size)
+{
+    hipError_t e;
+    size_t words = (size_t)num_tiles * (mode == kModeTable ? (size_t)(table_size + 1) : 1u);
+    if (s->state_words < words) {
+        size_t cap_bytes = s->state_words * 8;
+        e = grow((void **)&s->d_state, &cap_bytes, words * 8, true, s->stream);
+        if (e != hipSuccess)
+            return e;
+        s->state_words = cap_bytes / 8;
+    }
+    if (s->info_tiles < (size_t)num_tiles + 1) {
+        size_t cap_bytes = s->info_tiles * sizeof(TileInfo);
+        e = grow((void **)&s->d_info, &cap_bytes, ((size_t)num_tiles + 1) * sizeof(TileInfo), false,
+                 s->stream);
+        if (e != hipSuccess)
+            return e;
+        s->info_tiles = cap_bytes / sizeof(TileInfo);
+    }
+    if (mode == kModeBitmap) {
+        size_t need = (size_t)num_tiles * (kTile / 32);
+        if (s->bitmap_words < need) {
+            size_t cap_bytes = s->bitmap_words * 4;
+            e = grow((void **)&s->d_bitmap, &cap_bytes, need * 4, false, s->stream);
+            if (e != hipSuccess)
+                return e;
+            s->bitmap_words = cap_bytes / 4;
+        }
+    }
+    return hipSuccess;
+}
+
+hipError_t upload_pattern(Session *s, const uint8_t *pat, int L, const uint8_t *rep, int r)
+{
+    const size_t need = (size_t)L + (size_t)r + 16;
+    hipError_t e = grow((void **)&s->d_patrep, &s->patrep_cap, need, false, s->stream);
+    if (e != hipSuccess)
+        return e;
+    e = grow_host(&s->h_patrep, &s->h_patrep_cap, need);
+    if (e != hipSuccess)
+        return e;
+    if (L > 0)
+        memcpy(s->h_patrep, pat, (size_t)L);
+    if (r > 0)
+        memcpy(s->h_patrep + L, rep, (size_t)r);
+    if (L + r > 0)
+        return hipMemcpyAsync(s->d_patrep, s->h_patrep, (size_t)L + (size_t)r, hipMemcpyHostToDevice,
+                              s->stream);
+    return hipSuccess;
+}
+
+int64_t floor_from_seed(long long seed, int L)
+{
+    if (seed > (long long)(INT64_MAX - L))
+        return INT64_MAX;
+    const int64_t f = (int64_t)seed + L;
+    return f < 0 ? 0 : f;
+}
+
+struct PassResult {
+    int64_t kept = 0;
+    int64_t last = -1;
+    int64_t out_len = 0;
+};
+
+// Core pass: d_src (n bytes) -> d_dst.  dst_cap_fn(needed) must make d_dst hold at
+// least `needed` bytes and return its address (or nullptr with an error).
+// fixed_cap < 0: the destination can grow to any size; otherwise its capacity.
+template <class DstFn>
+int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, int L, const uint8_t *rep,
+             int r, int64_t floor_pos, int64_t fixed_cap, DstFn &&dst_for, PassResult *res, float *kernel_ms,
+             int *mode_out, ReplaceParams *params_out)
+{
+    CK(ensure_misc(s));
+    int table_size = 0;
+    const int mode = pattern_mode(pat, L, &table_size);
+    if (mode_out)
+        *mode_out = mode;
+
+    if (mode == kModeEmpty) {
+        const int64_t out = n + (n + 1) * (int64_t)r;
+        uint8_t *dst = nullptr;
+        int rc = dst_for(out, &dst);
+        if (rc != 0)
+            return rc;
+        res->kept = n + 1;
+        res->last = n;
+        res->out_len = out;
+        if (r == 0) {
+            if (n > 0 && dst != d_src)
+                CK(hipMemcpyAsync(dst, d_src, (size_t)n, hipMemcpyDeviceToDevice, s->stream));
+        } else {
+            CK(upload_pattern(s, pat, 0, rep, r));
+            ReplaceParams p{};
+            p.src = d_src;
+            p.dst = dst;
+            p.n = n;
+            p.rep = s->d_patrep;
+            p.r = r;
+            if (kernel_ms)
+                CK(hipEventRecord(s->ev0, s->stream));
+            CK(launch_empty_pattern(p, s->stream));
+            if (kernel_ms)
+                CK(hipEventRecord(s->ev1, s->stream));
+        }
+        CK(hipStreamSynchronize(s->stream));
+        if (kernel_ms) {
+            *kernel_ms = 0.f;
+            if (r != 0)
+                CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
+        }
+        return 0;
+    }
+
+    if (n == 0) {
+        uint8_t *dst = nullptr;
+        int rc = dst_for(0, &dst);
+        if (rc != 0)
+            return rc;
+        res->kept = 0;
+        res->last = -1;
+        res->out_len = 0;
+        if (kernel_ms)
+            *kernel_ms = 0.f;
+        return 0;
+    }
+
+    CK(upload_pattern(s, pat, L, rep, r));
+    const int64_t num_tiles = (n + kTile - 1) / kTile;
+    CK(ensure_scratch(s, num_tiles, mode, table_size));
+
+    ReplaceParams p{};
+    p.src = d_src;
+    p.n = n;
+    p.floor = floor_pos;
+    p.out_len = -1;
+    p.pat = s->d_patrep;
+    p.rep = s->d_patrep + L;
+    p.L = L;
+    p.r = r;
+    uint32_t pat4 = 0, mask4 = 0;
+    for (int i = 0; i < 4 && i < L; ++i) {
+        pat4 |= (uint32_t)pat[i] << (8 * i);
+        mask4 |= 0xffu << (8 * i);
+    }
+    p.pat4 = pat4;
+    p.mask4 = mask4;
+    p.state = s->d_state;
+    p.info = s->d_info;
+    p.bitmap = s->d_bitmap;
+    p.ticket = s->d_ticket;
+    p.stats = s->d_stats;
+    p.num_tiles = num_tiles;
+
+    auto launch_main = [&](int write) -> hipError_t {
+        hipError_t e = next_epoch(s);
+        if (e != hipSuccess)
+            return e;
+        p.epoch = s->epoch;
+        p.write_output = write;
+        p.ticket_base = s->ticket_base;
+        if (mode == kModeBorderFree)
+            e = launch_border_free(p, s->stream);
+        else if (mode == kModeTable)
+            e = launch_table(p, table_size, s->stream);
+        else
+            e = launch_bitmap_scatter(p, s->stream);
+        if (e == hipSuccess)
+            s->ticket_base += (uint64_t)num_tiles;   // every workgroup takes one ticket
+        return e;
+    };
+    auto read_stats = [&](int64_t *kept, int64_t *last) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost,
+                                      s->stream);
+        if (e != hipSuccess)
+            return e;
+        e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess)
+            return e;
+        const LaunchStats *h = (const LaunchStats *)(s->h_misc + 64);
+        const uint64_t kt = h->k_total, lk = h->last_kept;
+        if ((uint32_t)(h->pad[0] >> kEpochShift) == s->epoch)
+            return hipErrorUnknown;   // the kernel reported an inconsistency
+        if ((uint32_t)(kt >> kEpochShift) != s->epoch)
+            return hipErrorUnknown;   // the final tile did not report
+        *kept = (int64_t)(kt & ((1ull << kEpochShift) - 1));
+        *last = ((uint32_t)(lk >> kEpochShift) == s->epoch) ? (int64_t)(lk & ((1ull << kEpochShift) - 1)) - 1 : -1;
+        return hipSuccess;
+    };
+
+    const int64_t diff = (int64_t)r - L;
+    int64_t exact = -1;
+    int64_t kept = 0, last = -1;
+
+    if (mode == kModeBitmap) {
+        // candidates -> serial greedy (gives the exact count) -> scatter
+        CK(next_epoch(s));
+        p.epoch = s->epoch;
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev0, s->stream));
+        CK(launch_bitmap_mark(p, s->stream));
+        CK(launch_bitmap_greedy(p, s->stream));
+        CK(read_stats(&kept, &last));
+        exact = n + kept * diff;
+        uint8_t *dst = nullptr;
+        int rc = dst_for(exact, &dst);
+        if (rc != 0) {
+            res->out_len = exact;
+            return rc;
+        }
+        p.dst = dst;
+        p.out_len = exact;
+        float pre_ms = 0.f;
+        if (kernel_ms) {
+            CK(hipEventRecord(s->ev1, s->stream));
+            CK(hipEventSynchronize(s->ev1));
+            CK(hipEventElapsedTime(&pre_ms, s->ev0, s->ev1));
+            CK(hipEventRecord(s->ev0, s->stream));
+        }
+        p.write_output = 1;
+        p.ticket_base = s->ticket_base;
+        CK(launch_bitmap_scatter(p, s->stream));
+        s->ticket_base += (uint64_t)num_tiles;   // every workgroup takes one ticket
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev1, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        if (kernel_ms) {
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            *kernel_ms = pre_ms + ms;
+        }
+    } else {
+        // output size: shrink never grows; expand uses the non-overlapping worst
+        // case when affordable, otherwise a count-only pass
+        int64_t need = n;
+        if (diff > 0) {
+            const int64_t span = floor_pos < n ? n - floor_pos : 0;
+            const int64_t kmax = span / L;
+            const int64_t worst = n + kmax * diff;
+            int64_t afford = 4 * n > (256ll << 20) ? 4 * n : (256ll << 20);
+            if (fixed_cap >= 0)
+                afford = fixed_cap;
+            if (worst <= afford) {
+                need = worst;
+            } else {
+                CK(launch_main(0));
+                CK(read_stats(&kept, &last));
+                need = n + kept * diff;
+                exact = need;
+            }
+        }
+        uint8_t *dst = nullptr;
+        int rc = dst_for(need, &dst);
+        if (rc != 0) {
+            res->out_len = exact >= 0 ? exact : need;
+            return rc;
+        }
+        p.dst = dst;
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev0, s->stream));
+        CK(launch_main(1));
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev1, s->stream));
+        CK(read_stats(&kept, &last));
+        if (kernel_ms)
+            CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
+    }
+    res->kept = kept;
+    res->last = last;
+    res->out_len = n + kept * diff;
+    if (params_out) {
+        *params_out = p;
+        params_out->epoch = s->epoch;
+    }
+    return 0;
+}
+
+// session apply: d_buf -> d_tmp, swap
+int session_apply(Session *s, const uint8_t *pat, int L, const uint8_t *rep, int r, int64_t floor_pos,
+                  size_t *new_len, long long *out_last)
+{
+    if ((L > 0 && !pat) || (r > 0 && !rep) || L < 0 || r < 0)
+        return -1;
+    PassResult res;
+    int mode = 0;
+    ReplaceParams qp{};
+    auto dst_for = [&](int64_t need, uint8_t **dst) -> int {
+        hipError_t e = grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(need > 0 ? need : 1), false, s->stream);
+        if (e != hipSuccess)
+            return (int)e;
+        *dst = s->d_tmp;
+        return 0;
+    };
+    s->q_valid = false;
+    int rc = run_pass(s, s->d_buf, (int64_t)s->len, pat, L, rep, r, floor_pos, -1, dst_for, &res, nullptr,
+                      &mode, &qp);
+    if (rc != 0)
+        return rc;
+    std::swap(s->d_buf, s->d_tmp);
+    std::swap(s->cap_buf, s->cap_tmp);
+    const size_t src_len = s->len;
+    s->len = (size_t)res.out_len;
+    // the previous contents now live in d_tmp: queries recompute from there
+    s->q_valid = true;
+    s->q_mode = mode;
+    s->q_params = qp;
+    s->q_params.src = s->d_tmp;
+    s->q_params.n = (int64_t)src_len;
+    if (new_len)
+        *new_len = s->len;
+    if (out_last)
+        *out_last = res.last;
+    return 0;
+}
+
+int session_query(Session *s, size_t limit, int64_t *count, int64_t *last)
+{
+    *count = 0;
+    *last = -1;
+    if (!s->q_valid)
+        return 0;
+    const int64_t n = s->q_params.n;
+    const int64_t lim = (int64_t)limit < n ? (int64_t)limit : n;
+    if (lim <= 0)
+        return 0;
+    if (s->q_mode == kModeEmpty) {
+        *count = lim;
+        *last = lim - 1;
+        return 0;
+    }
+    CK(launch_query(s->q_params, s->q_mode, lim, s->d_query, s->stream));
+    CK(hipMemcpyAsync(s->h_misc + 128, s->d_query, 16, hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    const int64_t *h = (const int64_t *)(s->h_misc + 128);
+    *count = h[0];
+    *last = h[1];
+    return 0;
+}
+
+int session_upload(Session *s, const uint8_t *src, size_t len)
+{
+    // over-read slack for full 16-byte tile loads is not needed (loads are bounded)
+    CK(grow((void **)&s->d_buf, &s->cap_buf, len > 0 ? len : 1, false, s->stream));
+    if (len > 0)
+        CK(hipMemcpyAsync(s->d_buf, src, len, hipMemcpyHostToDevice, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    s->len = len;
+    s->q_valid = false;
+    return 0;
+}
+
+void destroy(Session *s)
+{
+    if (!s)
+        return;
+    DeviceGuard g(s->device);
+    if (s->stream)
+        (void)hipStreamSynchronize(s->stream);
+    void *dev[] = {s->d_buf, s->d_tmp, s->d_state, s->d_info, s->d_bitmap, s->d_patrep, s->d_presence, s->d_misc};
+    for (void *p : dev)
+        if (p)
+            (void)hipFree(p);
+    if (s->h_patrep)
+        (void)hipHostFree(s->h_patrep);
+    if (s->h_misc)
+        (void)hipHostFree(s->h_misc);
+    if (s->ev0)
+        (void)hipEventDestroy(s->ev0);
+    if (s->ev1)
+        (void)hipEventDestroy(s->ev1);
+    if (s->stream)
+        (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int open_session(Session **out)
+{
+    Session *s = new (std::nothrow) Session();
+    if (!s)
+        return -2;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        delete s;
+        return (int)e;
+    }
+    s->device = dev;
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&s->ev1);
+    if (e != hipSuccess) {
+        destroy(s);
+        return (int)e;
+    }
+    int rc = (int)ensure_misc(s);
+    if (rc != 0) {
+        destroy(s);
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+} // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+const char *cuda_replace_version(void) { return "cuda_replace_amd 1.0 gfx950 single-pass"; }
+
+int cuda_replace_open(void **ph, const uint8_t *src, size_t len)
+{
+    if (!ph || (len > 0 && !src))
+        return -1;
+    *ph = nullptr;
+    Session *s = nullptr;
+    int rc = open_session(&s);
+    if (rc != 0)
+        return rc;
+    rc = session_upload(s, src, len);
+    if (rc != 0) {
+        destroy(s);
+        return rc;
+    }
+    *ph = s;
+    return 0;
+}
+
+int cuda_replace_reset(void *h, const uint8_t *src, size_t len)
+{
+    if (!h || (len > 0 && !src))
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    return session_upload(s, src, len);
+}
+
+int cuda_replace_build_index(void *h, size_t chunk_size)
+{
+    if (!h)
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    if (chunk_size == 0)
+        chunk_size = 1u << 20;
+    if (s->len == 0)
+        return 0;
+    const int64_t chunks = (int64_t)((s->len + chunk_size - 1) / chunk_size);
+    size_t cap_bytes = s->presence_words * 8;
+    CK(grow((void **)&s->d_presence, &cap_bytes, (size_t)chunks * 32, false, s->stream));
+    s->presence_words = cap_bytes / 8;
+    CK(launch_presence_index(s->d_buf, (int64_t)s->len, (int64_t)chunk_size, s->d_presence, chunks, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int cuda_replace_apply(void *h, const uint8_t *pat, int pat_len, const uint8_t *rep, int rep_len,
+                       size_t *new_len)
+{
+    if (!h)
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    return session_apply(s, pat, pat_len, rep, rep_len, 0, new_len, nullptr);
+}
+
+int cuda_replace_apply_seeded(void *h, const uint8_t *pat, int pat_len, const uint8_t *rep, int rep_len,
+                              long long prev_last_rel, long long *out_last_rel, size_t *new_len)
+{
+    if (!h)
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    const int64_t f = pat_len > 0 ? floor_from_seed(prev_last_rel, pat_len) : 0;
+    return session_apply(s, pat, pat_len, rep, rep_len, f, new_len, out_last_rel);
+}
+
+int cr_query_prefix64(void *h, size_t limit, long long *kept_before_limit, long long *last_kept_before_limit)
+{
+    if (!h)
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    int64_t c = 0, l = -1;
+    int rc = session_query(s, limit, &c, &l);
+    if (rc != 0)
+        return rc;
+    if (kept_before_limit)
+        *kept_before_limit = c;
+    if (last_kept_before_limit)
+        *last_kept_before_limit = l;
+    return 0;
+}
+
+int cuda_replace_query_prefix(void *h, size_t limit, int *kept_before_limit, int *last_kept_before_limit)
+{
+    long long c = 0, l = -1;
+    int rc = cr_query_prefix64(h, limit, &c, &l);
+    if (rc != 0)
+        return rc;
+    if (c > INT_MAX || l > INT_MAX)
+        return -4;
+    if (kept_before_limit)
+        *kept_before_limit = (int)c;
+    if (last_kept_before_limit)
+        *last_kept_before_limit = (int)l;
+    return 0;
+}
+
+int cuda_replace_apply_batch(void *h, const uint8_t **pats, const int *pat_lens, const uint8_t **reps,
+                             const int *rep_lens, int count, size_t *new_len)
+{
+    if (!h)
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    for (int i = 0; i < count; ++i) {
+        if (!pats || !pat_lens)
+            break;   // no patterns: every pair is skipped
+        const uint8_t *p = pats[i];
+        const int pl = pat_lens[i];
+        const uint8_t *r = (reps && rep_lens) ? reps[i] : nullptr;
+        const int rl = (reps && rep_lens) ? rep_lens[i] : 0;
+        int rc = session_apply(s, p, pl, r, rl, 0, nullptr, nullptr);
+        if (rc != 0)
+            return rc;
+    }
+    if (new_len)
+        *new_len = s->len;
+    return 0;
+}
+
+int cuda_replace_result(void *h, uint8_t **out_host, size_t *outlen_host)
+{
+    if (!h || !out_host || !outlen_host)
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    *out_host = nullptr;
+    *outlen_host = s->len;
+    if (s->len == 0)
+        return 0;
+    void *hp = nullptr;
+    hipError_t e = hipHostMalloc(&hp, s->len, hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hp, s->d_buf, s->len, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        if (hp)
+            (void)hipHostFree(hp);
+        *out_host = nullptr;
+        *outlen_host = 0;
+        return (int)e;
+    }
+    *out_host = (uint8_t *)hp;
+    return 0;
+}
+
+void cuda_free_host(void *p)
+{
+    if (p)
+        (void)hipHostFree(p);
+}
+
+void cuda_replace_close(void *h)
+{
+    if (!h)
+        return;
+    Session *s = (Session *)h;
+    {
+        std::lock_guard<std::mutex> lock(s->mtx);
+        if (!s->alive)
+            return;
+        s->alive = false;
+    }
+    destroy(s);
+}
+
+int cuda_replace_unified(const uint8_t *src, size_t src_len, const uint8_t *pat, int pat_len, const uint8_t *rep,
+                         int rep_len, uint8_t **out_host, size_t *out_len)
+{
+    if (!out_host || !out_len)
+        return -1;
+    *out_host = nullptr;
+    *out_len = 0;
+    void *h = nullptr;
+    int rc = cuda_replace_open(&h, src, src_len);
+    if (rc != 0)
+        return rc;
+    rc = cuda_replace_apply(h, pat, pat_len, rep, rep_len, out_len);
+    if (rc == 0)
+        rc = cuda_replace_result(h, out_host, out_len);
+    cuda_replace_close(h);
+    return rc;
+}
+
+int cr_session_length(void *h, size_t *len)
+{
+    if (!h || !len)
+        return -1;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    *len = s->len;
+    return 0;
+}
+
+int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d_dst, size_t dst_cap, const uint8_t *pat,
+                      int pat_len, const uint8_t *rep, int rep_len, long long seed, size_t *out_len,
+                      long long *out_last, float *kernel_ms)
+{
+    if (!h || (n > 0 && !d_src) || (pat_len > 0 && !pat) || (rep_len > 0 && !rep) || pat_len < 0 || rep_len < 0)
+        return -1;
+    if (((uintptr_t)d_src & 15u) || ((uintptr_t)d_dst & 15u))
+        return (int)hipErrorInvalidValue;
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    const int64_t f = pat_len > 0 ? floor_from_seed(seed, pat_len) : 0;
+    PassResult res;
+    auto dst_for = [&](int64_t need, uint8_t **dst) -> int {
+        if ((size_t)need > dst_cap || (need > 0 && !d_dst))
+            return (int)hipErrorInvalidValue;
+        *dst = d_dst;
+        return 0;
+    };
+    int rc = run_pass(s, d_src, (int64_t)n, pat, pat_len, rep, rep_len, f, (int64_t)dst_cap, dst_for, &res,
+                      kernel_ms, nullptr, nullptr);
+    if (out_len)
+        *out_len = (size_t)res.out_len;
+    if (rc != 0)
+        return rc;
+    if (out_last)
+        *out_last = res.last;
+    return 0;
+}
+
+int cr_pattern_mode(const uint8_t *pat, int pat_len)
+{
+    if (pat_len < 0 || (pat_len > 0 && !pat))
+        return -1;
+    int m = pattern_mode(pat, pat_len, nullptr);
+    return m == kModeEmpty ? 0 : m;
+}
+
+} // extern "C"

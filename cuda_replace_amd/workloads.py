@@ -1,0 +1,110 @@
+"""Seeded synthetic inputs for the benchmark configurations of the replace path.
+
+Every generator is deterministic for a given seed (numpy PCG64 streams), so the
+GPU box regenerates byte-identical inputs and the committed sha256 fixtures in
+``tests/golden/`` apply there too.
+
+Workloads (sizes are the full benchmark sizes; tests pass smaller ``n``):
+
+* ``uniform_abcd``  -- 64 MiB, each byte uniform over ``b"abcd"`` (seed 0);
+  ``b"ab" -> b"X"`` (shrink, ~1/16 match density).
+* ``all_a``         -- ``b"a" * n``; ``b"aa" -> b"X"`` (maximal overlap).
+* ``zipf_text``     -- lowercase words (50k-word seeded vocabulary, lengths
+  2-12, Zipf(1.1) rank frequencies), separated by space (90%) or newline (10%).
+* ``text_fragment`` -- the 4-byte fragment of a frequent vocabulary word whose
+  density in a 4 MiB sample is closest to 1%; the expand replacement is 12
+  seeded uppercase bytes, the shrink replacement one byte.
+"""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+MiB = 1 << 20
+GiB = 1 << 30
+
+_VOCAB_SIZE = 50_000
+_ZIPF_S = 1.1
+_SEP_SPACE_FRAC = 0.9
+_INV_BITS = 22
+
+
+def uniform_abcd(n: int = 64 * MiB, seed: int = 0) -> bytes:
+    rng = np.random.default_rng(seed)
+    lut = np.frombuffer(b"abcd", dtype=np.uint8)
+    return lut[rng.integers(0, 4, size=n, dtype=np.uint8)].tobytes()
+
+
+def all_a(n: int) -> bytes:
+    return b"a" * n
+
+
+@functools.lru_cache(maxsize=4)
+def _vocab(seed: int):
+    rng = np.random.default_rng([seed, 0x5EED])
+    lens = rng.integers(2, 13, size=_VOCAB_SIZE)              # 2..12 letters
+    letters = rng.integers(ord("a"), ord("z") + 1, size=(_VOCAB_SIZE, 13), dtype=np.uint8)
+    ranks = np.arange(1, _VOCAB_SIZE + 1, dtype=np.float64)
+    w = ranks ** (-_ZIPF_S)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    # inverse-CDF table: a uniform 22-bit draw u maps to the first rank whose
+    # cumulative weight exceeds (u + 0.5) / 2^22
+    grid = (np.arange(1 << _INV_BITS, dtype=np.float64) + 0.5) / (1 << _INV_BITS)
+    inv = np.minimum(np.searchsorted(cdf, grid, side="right"), _VOCAB_SIZE - 1).astype(np.int32)
+    return lens, letters, inv
+
+
+def zipf_text(n: int, seed: int = 2, chunk_words: int = 4_000_000) -> bytes:
+    """Exactly ``n`` bytes of seeded Zipf-frequency text."""
+    lens, letters, inv = _vocab(seed)
+    rng = np.random.default_rng([seed, 0x7E47])
+    out = bytearray()
+    col = np.arange(13)
+    while len(out) < n:
+        m = chunk_words
+        idx = inv[rng.integers(0, 1 << _INV_BITS, size=m, dtype=np.int32)]
+        wl = lens[idx]
+        mat = letters[idx]                                   # (m, 13)
+        sep = np.where(rng.integers(0, 10, size=m, dtype=np.int8) < 9, 32, 10).astype(np.uint8)
+        mat[np.arange(m), wl] = sep
+        mask = col[None, :] <= wl[:, None]
+        out += mat[mask].tobytes()
+    del out[n:]
+    return bytes(out)
+
+
+def vocab_words(seed: int = 2, top: int = 400):
+    """The ``top`` most frequent vocabulary words (rank order)."""
+    lens, letters, _ = _vocab(seed)
+    return [letters[i, :lens[i]].tobytes() for i in range(top)]
+
+
+def text_fragment(sample: bytes, seed: int = 2, target: float = 0.01) -> bytes:
+    """4-byte fragment of a frequent word with density closest to ``target``."""
+    best, best_err = None, None
+    seen = set()
+    for w in vocab_words(seed):
+        for j in range(len(w) - 3):
+            f = w[j:j + 4]
+            if f in seen:
+                continue
+            seen.add(f)
+            d = sample.count(f) / max(1, len(sample))
+            err = abs(d - target)
+            if best_err is None or err < best_err:
+                best, best_err = f, err
+    return best
+
+
+def expand_replacement(seed: int = 2, length: int = 12) -> bytes:
+    rng = np.random.default_rng([seed, 0xE7A0])
+    return rng.integers(ord("A"), ord("Z") + 1, size=length, dtype=np.uint8).tobytes()
+
+
+def text_config(n: int, seed: int = 2):
+    """(text, fragment, expand_rep, shrink_rep) for the 1 GiB text workload."""
+    text = zipf_text(n, seed)
+    frag = text_fragment(text[: 4 * MiB], seed)
+    return text, frag, expand_replacement(seed), b"#"

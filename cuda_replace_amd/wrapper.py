@@ -1,0 +1,278 @@
+"""ctypes binding of libcuda_replace.so with the reference wrapper's API.
+
+Same classes, methods, argument meaning and errors as the reference's
+``cuda_replace_wrapper.py`` (``CudaReplaceLib`` py:35-141, ``Session`` py:145-297,
+``replace_unified`` py:299-301, ``gpu_replace_streaming`` py:303-366): every
+method raises ``RuntimeError("... rc=N")`` on a non-zero return code.
+
+Differences, all towards ``bytes.replace`` or speed:
+
+* inputs are passed to the library without an extra host copy;
+* ``Session.length()`` asks the library for the length instead of copying the
+  whole buffer back (``cr_session_length``);
+* ``gpu_replace_streaming`` moves a chunk's cut to the end of a kept match that
+  straddles the chunk end (the reference cuts at the chunk end and loses bytes,
+  py:355-364) and handles the empty pattern like ``bytes.replace``;
+* 64-bit offsets everywhere, so the 2^30 expansion guard of py:327-340 is not
+  needed.
+
+There is no CPU fallback: if the library is missing, ``CudaReplaceLib`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_LIB = os.path.join(_HERE, "libcuda_replace.so")
+
+c_size_t = C.c_size_t
+c_uint8_p = C.POINTER(C.c_uint8)
+c_void_p = C.c_void_p
+c_int_p = C.POINTER(C.c_int)
+c_longlong_p = C.POINTER(C.c_longlong)
+c_size_t_p = C.POINTER(c_size_t)
+c_uint8_pp = C.POINTER(c_uint8_p)
+
+# symbols the library must export (the reference ABI plus extensions)
+REQUIRED_SYMBOLS = (
+    "cuda_replace_unified", "cuda_free_host", "cuda_replace_open", "cuda_replace_reset",
+    "cuda_replace_build_index", "cuda_replace_apply", "cuda_replace_apply_seeded",
+    "cuda_replace_query_prefix", "cuda_replace_apply_batch", "cuda_replace_result",
+    "cuda_replace_close", "cuda_replace_version",
+    "cr_session_length", "cr_query_prefix64", "cr_replace_device", "cr_pattern_mode",
+)
+
+
+def _u8(b: bytes):
+    """Pointer to the bytes object's own buffer (no copy; NULs allowed)."""
+    return C.cast(C.c_char_p(b), c_uint8_p)
+
+
+def _load(path: str) -> C.CDLL:
+    path = os.fspath(path)
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"libcuda_replace.so not found at {path}; build it with "
+            f"`python -m cuda_replace_amd.build`")
+    return C.CDLL(path)
+
+
+class CudaReplaceLib:
+    """Loads libcuda_replace.so and binds its signatures (py:35-121)."""
+
+    def __init__(self, dll_path: Optional[str] = None):
+        self.path = os.fspath(dll_path) if dll_path else DEFAULT_LIB
+        self.lib = _load(self.path)
+        lib = self.lib
+        lib.cuda_replace_unified.argtypes = [c_uint8_p, c_size_t, c_uint8_p, C.c_int, c_uint8_p,
+                                             C.c_int, C.POINTER(c_uint8_p), c_size_t_p]
+        lib.cuda_replace_unified.restype = C.c_int
+        lib.cuda_free_host.argtypes = [c_void_p]
+        lib.cuda_free_host.restype = None
+        lib.cuda_replace_open.argtypes = [C.POINTER(c_void_p), c_uint8_p, c_size_t]
+        lib.cuda_replace_open.restype = C.c_int
+        lib.cuda_replace_build_index.argtypes = [c_void_p, c_size_t]
+        lib.cuda_replace_build_index.restype = C.c_int
+        lib.cuda_replace_apply.argtypes = [c_void_p, c_uint8_p, C.c_int, c_uint8_p, C.c_int, c_size_t_p]
+        lib.cuda_replace_apply.restype = C.c_int
+        lib.cuda_replace_apply_batch.argtypes = [c_void_p, c_uint8_pp, c_int_p, c_uint8_pp, c_int_p,
+                                                 C.c_int, c_size_t_p]
+        lib.cuda_replace_apply_batch.restype = C.c_int
+        lib.cuda_replace_result.argtypes = [c_void_p, C.POINTER(c_uint8_p), c_size_t_p]
+        lib.cuda_replace_result.restype = C.c_int
+        lib.cuda_replace_close.argtypes = [c_void_p]
+        lib.cuda_replace_close.restype = None
+        lib.cuda_replace_reset.argtypes = [c_void_p, c_uint8_p, c_size_t]
+        lib.cuda_replace_reset.restype = C.c_int
+        lib.cuda_replace_apply_seeded.argtypes = [c_void_p, c_uint8_p, C.c_int, c_uint8_p, C.c_int,
+                                                  C.c_longlong, c_longlong_p, c_size_t_p]
+        lib.cuda_replace_apply_seeded.restype = C.c_int
+        lib.cuda_replace_query_prefix.argtypes = [c_void_p, c_size_t, C.POINTER(C.c_int),
+                                                  C.POINTER(C.c_int)]
+        lib.cuda_replace_query_prefix.restype = C.c_int
+        lib.cuda_replace_version.argtypes = []
+        lib.cuda_replace_version.restype = C.c_char_p
+        lib.cr_session_length.argtypes = [c_void_p, c_size_t_p]
+        lib.cr_session_length.restype = C.c_int
+        lib.cr_query_prefix64.argtypes = [c_void_p, c_size_t, c_longlong_p, c_longlong_p]
+        lib.cr_query_prefix64.restype = C.c_int
+        lib.cr_replace_device.argtypes = [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_uint8_p,
+                                          C.c_int, c_uint8_p, C.c_int, C.c_longlong, c_size_t_p,
+                                          c_longlong_p, C.POINTER(C.c_float)]
+        lib.cr_replace_device.restype = C.c_int
+        lib.cr_pattern_mode.argtypes = [c_uint8_p, C.c_int]
+        lib.cr_pattern_mode.restype = C.c_int
+
+    def version(self) -> str:
+        return self.lib.cuda_replace_version().decode()
+
+    def pattern_mode(self, pat: bytes) -> int:
+        return int(self.lib.cr_pattern_mode(_u8(pat), len(pat)))
+
+    def unified(self, src: bytes, pat: bytes, rep: bytes) -> bytes:
+        """One-shot replace (py:124-141)."""
+        out_ptr = c_uint8_p()
+        out_len = c_size_t()
+        rc = self.lib.cuda_replace_unified(_u8(src), len(src), _u8(pat), len(pat), _u8(rep), len(rep),
+                                           C.byref(out_ptr), C.byref(out_len))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_unified failed rc={rc}")
+        try:
+            return C.string_at(out_ptr, out_len.value) if out_len.value else b""
+        finally:
+            self.lib.cuda_free_host(out_ptr)
+
+
+class Session:
+    """A device-resident buffer with replace passes (py:145-297).
+
+    Distinct sessions may be used from different threads concurrently; calls on
+    one session are serialized by the library."""
+
+    def __init__(self, lib: CudaReplaceLib, src: bytes):
+        self.lib = lib
+        self.h = c_void_p()
+        rc = lib.lib.cuda_replace_open(C.byref(self.h), _u8(src), len(src))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_open failed rc={rc}")
+
+    def __enter__(self) -> "Session":
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        self.close()
+
+    def reset(self, src: bytes):
+        rc = self.lib.lib.cuda_replace_reset(self.h, _u8(src), len(src))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_reset failed rc={rc}")
+
+    def build_index(self, chunk_size: Optional[int] = None):
+        rc = self.lib.lib.cuda_replace_build_index(self.h, 0 if chunk_size is None else int(chunk_size))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_build_index failed rc={rc}")
+
+    def apply(self, pat: bytes, rep: bytes) -> int:
+        new_len = c_size_t()
+        rc = self.lib.lib.cuda_replace_apply(self.h, _u8(pat), len(pat), _u8(rep), len(rep),
+                                             C.byref(new_len))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_apply failed rc={rc}")
+        return int(new_len.value)
+
+    def apply_seeded(self, pat: bytes, rep: bytes, prev_last_rel: int) -> Tuple[int, int]:
+        new_len = c_size_t()
+        last_rel = C.c_longlong(-1)
+        rc = self.lib.lib.cuda_replace_apply_seeded(self.h, _u8(pat), len(pat), _u8(rep), len(rep),
+                                                    C.c_longlong(int(prev_last_rel)), C.byref(last_rel),
+                                                    C.byref(new_len))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_apply_seeded failed rc={rc}")
+        return int(new_len.value), int(last_rel.value)
+
+    def apply_batch(self, pairs: Sequence[Tuple[bytes, bytes]]) -> int:
+        count = len(pairs)
+        if count == 0:
+            return self.length()
+        pats = (c_uint8_p * count)(*[_u8(p) for p, _ in pairs])
+        reps = (c_uint8_p * count)(*[_u8(r) for _, r in pairs])
+        pat_lens = (C.c_int * count)(*[len(p) for p, _ in pairs])
+        rep_lens = (C.c_int * count)(*[len(r) for _, r in pairs])
+        keep = [p for p, _ in pairs] + [r for _, r in pairs]   # noqa: F841 (buffers stay alive)
+        new_len = c_size_t()
+        rc = self.lib.lib.cuda_replace_apply_batch(self.h, C.cast(pats, c_uint8_pp), C.cast(pat_lens, c_int_p),
+                                                   C.cast(reps, c_uint8_pp), C.cast(rep_lens, c_int_p),
+                                                   count, C.byref(new_len))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_apply_batch failed rc={rc}")
+        return int(new_len.value)
+
+    def query_prefix(self, limit: int) -> Tuple[int, int]:
+        kept = C.c_longlong(0)
+        last = C.c_longlong(-1)
+        rc = self.lib.lib.cr_query_prefix64(self.h, int(limit), C.byref(kept), C.byref(last))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_query_prefix failed rc={rc}")
+        return int(kept.value), int(last.value)
+
+    def result(self) -> bytes:
+        out_ptr = c_uint8_p()
+        out_len = c_size_t()
+        rc = self.lib.lib.cuda_replace_result(self.h, C.byref(out_ptr), C.byref(out_len))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_result failed rc={rc}")
+        try:
+            return C.string_at(out_ptr, out_len.value) if out_len.value else b""
+        finally:
+            self.lib.lib.cuda_free_host(out_ptr)
+
+    def length(self) -> int:
+        n = c_size_t()
+        rc = self.lib.lib.cr_session_length(self.h, C.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"cuda_replace_result(len) failed rc={rc}")
+        return int(n.value)
+
+    def close(self):
+        if self.h:
+            self.lib.lib.cuda_replace_close(self.h)
+            self.h = c_void_p(None)
+
+
+def replace_unified(lib: CudaReplaceLib, src: bytes, pat: bytes, rep: bytes) -> bytes:
+    """One-shot convenience around cuda_replace_unified() (py:299-301)."""
+    return lib.unified(src, pat, rep)
+
+
+def gpu_replace_streaming(lib: CudaReplaceLib, src: bytes, pairs: Iterable[Tuple[bytes, bytes]],
+                          chunk_bytes: int = 256 * 1024 * 1024) -> bytes:
+    """Memory-bounded chunked replace with an exact carry between chunks
+    (protocol of py:305-366): each chunk is taken with an L-1 byte overlap,
+    applied with the previous chunk's last kept start as seed, and cut after the
+    kept matches that start before the chunk end."""
+    if chunk_bytes <= 0:
+        chunk_bytes = 1
+    out = src
+    with Session(lib, b"") as sess:
+        for pat, rep in pairs:
+            L = len(pat)
+            n = len(out)
+            pieces: List[bytes] = []
+            if L == 0:
+                # bytes.replace(b"", rep): rep before every byte and once at the end
+                pos = 0
+                while True:
+                    end = min(pos + chunk_bytes, n)
+                    sess.reset(out[pos:end])
+                    sess.apply(pat, rep)
+                    chunk_out = sess.result()
+                    if end < n:
+                        pieces.append(chunk_out[:len(chunk_out) - len(rep)])
+                        pos = end
+                    else:
+                        pieces.append(chunk_out)
+                        break
+                out = b"".join(pieces)
+                continue
+            diff = len(rep) - L
+            pos = 0
+            prev_last: Optional[int] = None
+            while pos < n:
+                base_end = min(pos + chunk_bytes, n)
+                take_end = min(base_end + L - 1, n)
+                sess.reset(out[pos:take_end])
+                seed = -L if prev_last is None else prev_last - pos
+                sess.apply_seeded(pat, rep, seed)
+                kept_before, last_before = sess.query_prefix(base_end - pos)
+                cut = base_end - pos
+                if last_before >= 0:
+                    cut = max(cut, last_before + L)
+                gpu_out = sess.result()
+                pieces.append(gpu_out[:cut + kept_before * diff])
+                if last_before >= 0:
+                    prev_last = pos + last_before
+                pos += cut
+            out = b"".join(pieces)
+    return out

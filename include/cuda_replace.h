@@ -1,0 +1,133 @@
+/*
+ * cuda_replace.h -- C ABI of the MI355X (gfx950) byte-pattern replace library
+ * (libcuda_replace.so).
+ *
+ * The first twelve entry points keep the names, signatures, ownership rules and
+ * return codes of the reference library so its ctypes wrapper binds this .so
+ * unchanged (reference: cuda_replace_wrapper.py:48-121 binds them).  The cr_*
+ * entry points are extensions with no counterpart in the reference.
+ *
+ * Semantics: leftmost, non-overlapping, binary-safe replacement identical to
+ * Python's bytes.replace (cuda_replace.cu:2-3), including the empty pattern
+ * (the replacement is inserted at all n+1 gaps; the reference leaves the input
+ * unchanged there).  Offsets and lengths are 64-bit throughout.
+ *
+ * Return codes: 0 ok; -1 null handle or argument; -2 session allocation failed
+ * (open); -3 closed handle; -4 a result does not fit the ABI's int; any
+ * positive value is a hipError_t (hipErrorInvalidValue=1, hipErrorOutOfMemory=2
+ * and hipErrorUnknown=999 equal the CUDA codes the reference returns).
+ *
+ * Threading: every handle call takes that handle's mutex, so calls on one handle
+ * serialize and distinct handles run concurrently, each on its own HIP stream.
+ * Use after close is undefined (cuda_replace.cu:50-55).  A handle remembers the
+ * device that was current at open and runs there.
+ */
+#ifndef CUDA_REPLACE_H
+#define CUDA_REPLACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define CR_API __attribute__((visibility("default")))
+#else
+#define CR_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One-shot: open, apply, result, close.  *out_host is a pinned buffer the
+ * caller frees with cuda_free_host (NULL when *out_len == 0).
+ * Replaces cuda_replace.cu:2579-2601. */
+CR_API int cuda_replace_unified(const uint8_t *src, size_t src_len,
+                         const uint8_t *pat, int pat_len,
+                         const uint8_t *rep, int rep_len,
+                         uint8_t **out_host, size_t *out_len);
+
+/* Frees a buffer returned by cuda_replace_unified / cuda_replace_result;
+ * NULL is ignored.  Replaces cuda_replace.cu:2559-2563. */
+CR_API void cuda_free_host(void *p);
+
+/* Opens a session holding a device-resident copy of src.
+ * Replaces cuda_replace.cu:2256-2293. */
+CR_API int cuda_replace_open(void **ph, const uint8_t *src, size_t len);
+
+/* Replaces the session's buffer with src (grows capacity if needed).
+ * Replaces cuda_replace.cu:2295-2318. */
+CR_API int cuda_replace_reset(void *h, const uint8_t *src, size_t len);
+
+/* Builds a per-chunk 256-bit byte-presence index (chunk_size 0 -> 1 MiB).  As in
+ * the reference, no apply consults it.  Replaces cuda_replace.cu:2320-2354. */
+CR_API int cuda_replace_build_index(void *h, size_t chunk_size);
+
+/* One replace pass over the session buffer; *new_len gets the new length.
+ * Replaces cuda_replace.cu:2356-2401. */
+CR_API int cuda_replace_apply(void *h, const uint8_t *pat, int pat_len,
+                       const uint8_t *rep, int rep_len, size_t *new_len);
+
+/* Apply with a carry: prev_last_rel is the last kept start of the previous
+ * chunk relative to this buffer (candidates i with i - prev_last_rel < pat_len
+ * are not kept; any value <= -pat_len means no carry).  *out_last_rel gets the
+ * last kept start of this pass or -1.  Replaces cuda_replace.cu:2403-2450. */
+CR_API int cuda_replace_apply_seeded(void *h, const uint8_t *pat, int pat_len,
+                              const uint8_t *rep, int rep_len,
+                              long long prev_last_rel, long long *out_last_rel,
+                              size_t *new_len);
+
+/* Kept starts of the most recent apply strictly below limit (clamped to that
+ * apply's source length), in that source's coordinates: their number and the
+ * last one (or -1).  Returns -4 if the count exceeds INT_MAX (see
+ * cr_query_prefix64).  Replaces cuda_replace.cu:2452-2463. */
+CR_API int cuda_replace_query_prefix(void *h, size_t limit,
+                              int *kept_before_limit, int *last_kept_before_limit);
+
+/* count sequential seedless applies.  NULL pats or pat_lens skips every pair;
+ * NULL reps or rep_lens means empty replacements.  Replaces
+ * cuda_replace.cu:2465-2527. */
+CR_API int cuda_replace_apply_batch(void *h, const uint8_t **pats, const int *pat_lens,
+                             const uint8_t **reps, const int *rep_lens,
+                             int count, size_t *new_len);
+
+/* Pinned host copy of the session buffer (free with cuda_free_host).
+ * Replaces cuda_replace.cu:2528-2557. */
+CR_API int cuda_replace_result(void *h, uint8_t **out_host, size_t *outlen_host);
+
+/* Closes a session; NULL is ignored.  Replaces cuda_replace.cu:2565-2577. */
+CR_API void cuda_replace_close(void *h);
+
+/* Library version string.  Replaces cuda_replace.cu:2251-2254. */
+CR_API const char *cuda_replace_version(void);
+
+/* ---- extensions (no counterpart in the reference) ------------------------ */
+
+/* Current session length without a device-to-host copy of the buffer (the
+ * reference wrapper's Session.length() copies the whole buffer). */
+CR_API int cr_session_length(void *h, size_t *len);
+
+/* 64-bit form of cuda_replace_query_prefix. */
+CR_API int cr_query_prefix64(void *h, size_t limit, long long *kept_before_limit,
+                      long long *last_kept_before_limit);
+
+/* Replace between caller-owned device buffers on the session's stream and
+ * scratch: d_src (n bytes, 16-byte aligned) -> d_dst (dst_cap bytes, 16-byte
+ * aligned, not overlapping d_src).  seed as in cuda_replace_apply_seeded
+ * (pass a value <= -pat_len for none).  *out_len gets the output length; if
+ * dst_cap is too small nothing is written, *out_len still gets the needed
+ * length and hipErrorInvalidValue is returned.  *kernel_ms (optional) gets the
+ * device time of the pass measured with HIP events on the session stream. */
+CR_API int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d_dst,
+                      size_t dst_cap, const uint8_t *pat, int pat_len,
+                      const uint8_t *rep, int rep_len, long long seed,
+                      size_t *out_len, long long *out_last, float *kernel_ms);
+
+/* Which suppression path a pattern takes: 1 border-free single pass, 2 carry
+ * tables (self-overlapping, length <= 16), 3 bitmap + serial greedy, 0 empty. */
+CR_API int cr_pattern_mode(const uint8_t *pat, int pat_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUDA_REPLACE_H */
