@@ -21,7 +21,7 @@ constexpr int kTile = kThreads * kBytesPerThread;   // 8192 source bytes per til
 constexpr int kHalo = 256;                    // LDS halo on each side of a tile
 constexpr int kMaxSinglePassPat = kHalo;      // single-pass kernels take L <= 256
 constexpr int kMaxTablePat = 16;              // carry tables (self-overlapping patterns) take L <= 16
-constexpr int kOutStage = 2 * kTile;          // LDS output staging window (bytes)
+constexpr int kStage = kTile + kTile / 8 + 64; // LDS stage for a tile's output (bytes)
 
 // ---- decoupled look-back state words ----------------------------------------
 // One 64-bit word per published value, written and read with relaxed agent-scope
@@ -56,8 +56,8 @@ enum Mode : int32_t {
 // stats words written by a launch
 struct LaunchStats {
     uint64_t k_total;      // epoch-tagged total kept count
-    uint64_t last_kept;    // epoch-tagged (last kept + 1), combined with atomicMax
-    uint64_t pad[2];       // pad[0]: epoch-tagged error code from the kernel (0 = none)
+    uint64_t last_kept;    // epoch-tagged (last kept + 1), written by the bitmap greedy
+    uint64_t pad[2];       // pad[1]: epoch-tagged abort flag of a look-back launch
 };
 
 struct ReplaceParams {
@@ -75,8 +75,6 @@ struct ReplaceParams {
     uint64_t *state;       // look-back words
     TileInfo *info;        // per-tile records (num_tiles)
     uint32_t *bitmap;      // candidate / kept bitmap (bitmap mode)
-    unsigned long long *ticket;  // dynamic tile ticket counter (monotonic)
-    uint64_t ticket_base;  // value of *ticket when this launch starts
     LaunchStats *stats;
     uint32_t epoch;        // 1..65535
     int32_t write_output;  // 0: count only
@@ -93,6 +91,7 @@ hipError_t launch_empty_pattern(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *ptr, uint64_t value, size_t count, hipStream_t s);
 hipError_t launch_query(const ReplaceParams &p, int mode, int64_t limit,
                         int64_t *out2, hipStream_t s);
+hipError_t launch_last_kept(const ReplaceParams &p, int64_t *out1, hipStream_t s);
 hipError_t launch_presence_index(const uint8_t *src, int64_t n, int64_t chunk,
                                  uint64_t *masks, int64_t num_chunks, hipStream_t s);
 

@@ -2,8 +2,7 @@
 //
 // A session keeps the buffer resident in HBM as a ping-pong pair (d_buf holds the
 // current contents, d_tmp receives the next pass) plus the scratch the single-pass
-// kernel needs: look-back state words, per-tile records, a ticket counter and a
-// small stats block.  Every apply is: upload pattern + replacement (pinned staging,
+// kernel needs: look-back state words, per-tile records and a small stats block.  Every apply is: upload pattern + replacement (pinned staging,
 // async), one kernel launch (two for long self-overlapping patterns, three for the
 // bitmap fallback), one 32-byte stats readback, swap.
 //
@@ -14,6 +13,7 @@
 #include "../../include/cuda_replace.h"
 
 #include <limits.h>
+#include <stdio.h>
 #include <mutex>
 #include <new>
 #include <stdlib.h>
@@ -26,6 +26,7 @@ namespace {
 
 constexpr size_t kAllocGranule = 2u << 20;   // device buffers grow in 2 MiB steps
 constexpr int kModeEmpty = 4;                // empty pattern (bytes.replace inserts)
+constexpr int kAborted = -1000;              // internal: a look-back launch gave up
 
 struct Session {
     std::mutex mtx;
@@ -53,13 +54,11 @@ struct Session {
     uint64_t *d_presence = nullptr;
     size_t presence_words = 0;
 
-    // small device block: ticket counter, stats, query output
+    // small device block: stats and query output
     uint8_t *d_misc = nullptr;
     uint8_t *h_misc = nullptr;       // pinned mirror
-    unsigned long long *d_ticket = nullptr;
     LaunchStats *d_stats = nullptr;
     int64_t *d_query = nullptr;
-    uint64_t ticket_base = 0;
     uint32_t epoch = 0;
 
     // record of the most recent apply, for prefix queries
@@ -199,10 +198,8 @@ hipError_t ensure_misc(Session *s)
     e = hipHostMalloc((void **)&s->h_misc, bytes, hipHostMallocDefault);
     if (e != hipSuccess)
         return e;
-    s->d_ticket = (unsigned long long *)s->d_misc;
     s->d_stats = (LaunchStats *)(s->d_misc + 64);
     s->d_query = (int64_t *)(s->d_misc + 128);
-    s->ticket_base = 0;
     return hipStreamSynchronize(s->stream);
 }
 
@@ -227,7 +224,10 @@ hipError_t next_epoch(Session *s)
 hipError_t ensure_scratch(Session *s, int64_t num_tiles, int mode, int table_size)
 {
     hipError_t e;
-    size_t words = (size_t)num_tiles * (mode == kModeTable ? (size_t)(table_size + 1) : 1u);
+    // one word (border-free) or one table of table_size words per tile, plus the
+    // fan-out-64 group entries above them
+    const size_t per = mode == kModeTable ? (size_t)table_size : 1u;
+    size_t words = per * ((size_t)num_tiles + (size_t)num_tiles / 32 + 8);
     if (s->state_words < words) {
         size_t cap_bytes = s->state_words * 8;
         e = grow((void **)&s->d_state, &cap_bytes, words * 8, true, s->stream);
@@ -291,27 +291,33 @@ struct PassResult {
 
 // Core pass: d_src (n bytes) -> d_dst.  dst_cap_fn(needed) must make d_dst hold at
 // least `needed` bytes and return its address (or nullptr with an error).
+// Core pass: d_src (n bytes) -> destination from dst_for(needed, &ptr).
 // fixed_cap < 0: the destination can grow to any size; otherwise its capacity.
+// want_last: also compute the last kept start (one extra small kernel).
 template <class DstFn>
 int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, int L, const uint8_t *rep,
-             int r, int64_t floor_pos, int64_t fixed_cap, DstFn &&dst_for, PassResult *res, float *kernel_ms,
-             int *mode_out, ReplaceParams *params_out)
+             int r, int64_t floor_pos, int64_t fixed_cap, bool want_last, DstFn &&dst_for, PassResult *res,
+             float *kernel_ms, int *mode_out, ReplaceParams *params_out)
 {
     CK(ensure_misc(s));
     int table_size = 0;
-    const int mode = pattern_mode(pat, L, &table_size);
-    if (mode_out)
-        *mode_out = mode;
+    int mode = pattern_mode(pat, L, &table_size);
 
     if (mode == kModeEmpty) {
+        if (mode_out)
+            *mode_out = mode;
         const int64_t out = n + (n + 1) * (int64_t)r;
         uint8_t *dst = nullptr;
         int rc = dst_for(out, &dst);
-        if (rc != 0)
+        if (rc != 0) {
+            res->out_len = out;
             return rc;
+        }
         res->kept = n + 1;
         res->last = n;
         res->out_len = out;
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev0, s->stream));
         if (r == 0) {
             if (n > 0 && dst != d_src)
                 CK(hipMemcpyAsync(dst, d_src, (size_t)n, hipMemcpyDeviceToDevice, s->stream));
@@ -323,22 +329,19 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             p.n = n;
             p.rep = s->d_patrep;
             p.r = r;
-            if (kernel_ms)
-                CK(hipEventRecord(s->ev0, s->stream));
             CK(launch_empty_pattern(p, s->stream));
-            if (kernel_ms)
-                CK(hipEventRecord(s->ev1, s->stream));
         }
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev1, s->stream));
         CK(hipStreamSynchronize(s->stream));
-        if (kernel_ms) {
-            *kernel_ms = 0.f;
-            if (r != 0)
-                CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
-        }
+        if (kernel_ms)
+            CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
         return 0;
     }
 
     if (n == 0) {
+        if (mode_out)
+            *mode_out = mode;
         uint8_t *dst = nullptr;
         int rc = dst_for(0, &dst);
         if (rc != 0)
@@ -353,7 +356,6 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
 
     CK(upload_pattern(s, pat, L, rep, r));
     const int64_t num_tiles = (n + kTile - 1) / kTile;
-    CK(ensure_scratch(s, num_tiles, mode, table_size));
 
     ReplaceParams p{};
     p.src = d_src;
@@ -371,98 +373,95 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     }
     p.pat4 = pat4;
     p.mask4 = mask4;
-    p.state = s->d_state;
-    p.info = s->d_info;
-    p.bitmap = s->d_bitmap;
-    p.ticket = s->d_ticket;
     p.stats = s->d_stats;
     p.num_tiles = num_tiles;
 
-    auto launch_main = [&](int write) -> hipError_t {
-        hipError_t e = next_epoch(s);
-        if (e != hipSuccess)
-            return e;
-        p.epoch = s->epoch;
-        p.write_output = write;
-        p.ticket_base = s->ticket_base;
-        if (mode == kModeBorderFree)
-            e = launch_border_free(p, s->stream);
-        else if (mode == kModeTable)
-            e = launch_table(p, table_size, s->stream);
-        else
-            e = launch_bitmap_scatter(p, s->stream);
-        if (e == hipSuccess)
-            s->ticket_base += (uint64_t)num_tiles;   // every workgroup takes one ticket
-        return e;
-    };
-    auto read_stats = [&](int64_t *kept, int64_t *last) -> hipError_t {
+    // read the launch stats; kAborted when a look-back launch gave up
+    auto read_stats = [&](int64_t *kept, int64_t *last) -> int {
         hipError_t e = hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost,
                                       s->stream);
         if (e != hipSuccess)
-            return e;
+            return (int)e;
         e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess)
-            return e;
+            return (int)e;
         const LaunchStats *h = (const LaunchStats *)(s->h_misc + 64);
+        if ((uint32_t)(h->pad[1] >> kEpochShift) == s->epoch) {
+            if (getenv("CR_DEBUG"))
+                fprintf(stderr, "[cr] look-back gave up (mode %d, %lld tiles): re-running on the bitmap path\n",
+                        mode, (long long)num_tiles);
+            return kAborted;
+        }
         const uint64_t kt = h->k_total, lk = h->last_kept;
-        if ((uint32_t)(h->pad[0] >> kEpochShift) == s->epoch)
-            return hipErrorUnknown;   // the kernel reported an inconsistency
         if ((uint32_t)(kt >> kEpochShift) != s->epoch)
-            return hipErrorUnknown;   // the final tile did not report
+            return (int)hipErrorUnknown;   // the final tile did not report
         *kept = (int64_t)(kt & ((1ull << kEpochShift) - 1));
-        *last = ((uint32_t)(lk >> kEpochShift) == s->epoch) ? (int64_t)(lk & ((1ull << kEpochShift) - 1)) - 1 : -1;
-        return hipSuccess;
+        if (last)
+            *last = ((uint32_t)(lk >> kEpochShift) == s->epoch)
+                        ? (int64_t)(lk & ((1ull << kEpochShift) - 1)) - 1
+                        : -1;
+        return 0;
     };
 
     const int64_t diff = (int64_t)r - L;
-    int64_t exact = -1;
     int64_t kept = 0, last = -1;
 
-    if (mode == kModeBitmap) {
-        // candidates -> serial greedy (gives the exact count) -> scatter
-        CK(next_epoch(s));
-        p.epoch = s->epoch;
-        if (kernel_ms)
-            CK(hipEventRecord(s->ev0, s->stream));
-        CK(launch_bitmap_mark(p, s->stream));
-        CK(launch_bitmap_greedy(p, s->stream));
-        CK(read_stats(&kept, &last));
-        exact = n + kept * diff;
-        uint8_t *dst = nullptr;
-        int rc = dst_for(exact, &dst);
-        if (rc != 0) {
-            res->out_len = exact;
-            return rc;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (attempt == 1)
+            mode = kModeBitmap;   // look-back gave up: the bitmap path needs no tile order
+        CK(ensure_scratch(s, num_tiles, mode, table_size));
+        p.state = s->d_state;
+        p.info = s->d_info;
+        p.bitmap = s->d_bitmap;
+
+        auto launch_main = [&](int write) -> hipError_t {
+            hipError_t e = next_epoch(s);
+            if (e != hipSuccess)
+                return e;
+            p.epoch = s->epoch;
+            p.write_output = write;
+            if (mode == kModeBorderFree)
+                return launch_border_free(p, s->stream);
+            return launch_table(p, table_size, s->stream);
+        };
+
+        if (mode == kModeBitmap) {
+            // candidates -> serial greedy (gives the exact count) -> scatter
+            CK(next_epoch(s));
+            p.epoch = s->epoch;
+            if (kernel_ms)
+                CK(hipEventRecord(s->ev0, s->stream));
+            CK(launch_bitmap_mark(p, s->stream));
+            CK(launch_bitmap_greedy(p, s->stream));
+            int rc = read_stats(&kept, &last);
+            if (rc != 0)
+                return rc;
+            const int64_t exact = n + kept * diff;
+            uint8_t *dst = nullptr;
+            rc = dst_for(exact, &dst);
+            if (rc != 0) {
+                res->out_len = exact;
+                return rc;
+            }
+            p.dst = dst;
+            p.out_len = exact;
+            p.write_output = 1;
+            CK(launch_bitmap_scatter(p, s->stream));
+            if (kernel_ms)
+                CK(hipEventRecord(s->ev1, s->stream));
+            CK(hipStreamSynchronize(s->stream));
+            if (kernel_ms)
+                CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
+            break;
         }
-        p.dst = dst;
-        p.out_len = exact;
-        float pre_ms = 0.f;
-        if (kernel_ms) {
-            CK(hipEventRecord(s->ev1, s->stream));
-            CK(hipEventSynchronize(s->ev1));
-            CK(hipEventElapsedTime(&pre_ms, s->ev0, s->ev1));
-            CK(hipEventRecord(s->ev0, s->stream));
-        }
-        p.write_output = 1;
-        p.ticket_base = s->ticket_base;
-        CK(launch_bitmap_scatter(p, s->stream));
-        s->ticket_base += (uint64_t)num_tiles;   // every workgroup takes one ticket
-        if (kernel_ms)
-            CK(hipEventRecord(s->ev1, s->stream));
-        CK(hipStreamSynchronize(s->stream));
-        if (kernel_ms) {
-            float ms = 0.f;
-            CK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-            *kernel_ms = pre_ms + ms;
-        }
-    } else {
+
         // output size: shrink never grows; expand uses the non-overlapping worst
-        // case when affordable, otherwise a count-only pass
+        // case when affordable, otherwise a count-only pass first
         int64_t need = n;
+        int64_t exact = -1;
         if (diff > 0) {
             const int64_t span = floor_pos < n ? n - floor_pos : 0;
-            const int64_t kmax = span / L;
-            const int64_t worst = n + kmax * diff;
+            const int64_t worst = n + (span / L) * diff;
             int64_t afford = 4 * n > (256ll << 20) ? 4 * n : (256ll << 20);
             if (fixed_cap >= 0)
                 afford = fixed_cap;
@@ -470,7 +469,11 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                 need = worst;
             } else {
                 CK(launch_main(0));
-                CK(read_stats(&kept, &last));
+                int rc = read_stats(&kept, nullptr);
+                if (rc == kAborted)
+                    continue;
+                if (rc != 0)
+                    return rc;
                 need = n + kept * diff;
                 exact = need;
             }
@@ -487,10 +490,24 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         CK(launch_main(1));
         if (kernel_ms)
             CK(hipEventRecord(s->ev1, s->stream));
-        CK(read_stats(&kept, &last));
+        if (want_last)
+            CK(launch_last_kept(p, s->d_query, s->stream));
+        rc = read_stats(&kept, nullptr);
+        if (rc == kAborted)
+            continue;
+        if (rc != 0)
+            return rc;
+        if (want_last) {
+            CK(hipMemcpyAsync(s->h_misc + 128, s->d_query, 8, hipMemcpyDeviceToHost, s->stream));
+            CK(hipStreamSynchronize(s->stream));
+            last = ((const int64_t *)(s->h_misc + 128))[0];
+        }
         if (kernel_ms)
             CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
+        break;
     }
+    if (mode_out)
+        *mode_out = mode;
     res->kept = kept;
     res->last = last;
     res->out_len = n + kept * diff;
@@ -518,8 +535,8 @@ int session_apply(Session *s, const uint8_t *pat, int L, const uint8_t *rep, int
         return 0;
     };
     s->q_valid = false;
-    int rc = run_pass(s, s->d_buf, (int64_t)s->len, pat, L, rep, r, floor_pos, -1, dst_for, &res, nullptr,
-                      &mode, &qp);
+    int rc = run_pass(s, s->d_buf, (int64_t)s->len, pat, L, rep, r, floor_pos, -1, out_last != nullptr, dst_for,
+                      &res, nullptr, &mode, &qp);
     if (rc != 0)
         return rc;
     std::swap(s->d_buf, s->d_tmp);
@@ -879,8 +896,8 @@ int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d_dst, s
         *dst = d_dst;
         return 0;
     };
-    int rc = run_pass(s, d_src, (int64_t)n, pat, pat_len, rep, rep_len, f, (int64_t)dst_cap, dst_for, &res,
-                      kernel_ms, nullptr, nullptr);
+    int rc = run_pass(s, d_src, (int64_t)n, pat, pat_len, rep, rep_len, f, (int64_t)dst_cap, out_last != nullptr,
+                      dst_for, &res, kernel_ms, nullptr, nullptr);
     if (out_len)
         *out_len = (size_t)res.out_len;
     if (rc != 0)
