@@ -167,7 +167,6 @@ def main() -> int:
     del cp
 
     out_len = C.c_size_t()
-    last = C.c_longlong()
     kms = C.c_float()
 
     def run(name, key, pat, rep):
@@ -175,7 +174,7 @@ def main() -> int:
         rc = lib.lib.cr_replace_device(h, C.c_void_p(src.data_ptr()), n, C.c_void_p(dst.data_ptr()),
                                        dst.numel(), C.cast(C.c_char_p(pat), C.POINTER(C.c_uint8)), len(pat),
                                        C.cast(C.c_char_p(rep), C.POINTER(C.c_uint8)), len(rep), -len(pat),
-                                       C.byref(out_len), C.byref(last), C.byref(kms))
+                                       C.byref(out_len), None, C.byref(kms))
         if rc != 0:
             raise RuntimeError(f"cr_replace_device({name}) rc={rc}")
         return out_len.value, kms.value

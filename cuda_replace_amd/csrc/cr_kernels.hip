@@ -1267,6 +1267,13 @@ hipError_t launch_tile(const ReplaceParams &p, hipStream_t s)
         hipLaunchKernelGGL((tile_kernel<MODE, LT>), dim3((unsigned)grid), dim3(kThreads), 0, s, pc);
         return hipGetLastError();
     }
+    // CR_COOPERATIVE=0: plain launch of the same (resident-sized) grid; rocprofv3
+    // crashes at process exit after cooperative launches and loses its trace
+    static const bool coop = !(getenv("CR_COOPERATIVE") && getenv("CR_COOPERATIVE")[0] == '0');
+    if (!coop) {
+        hipLaunchKernelGGL((tile_kernel<MODE, LT>), dim3((unsigned)grid), dim3(kThreads), 0, s, pc);
+        return hipGetLastError();
+    }
     void *args[] = {&pc};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&tile_kernel<MODE, LT>), dim3((unsigned)grid),
                                       dim3(kThreads), args, 0, s);
