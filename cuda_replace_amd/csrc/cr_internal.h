@@ -18,24 +18,20 @@ namespace cr {
 constexpr int kThreads = 256;                 // 4 wave64 per workgroup
 constexpr int kBytesPerThread = 32;           // one thread owns 32 consecutive source bytes
 constexpr int kTile = kThreads * kBytesPerThread;   // 8192 source bytes per tile
-constexpr int kHalo = 256;                    // LDS halo on each side of a tile
-constexpr int kMaxSinglePassPat = kHalo;      // single-pass kernels take L <= 256
+constexpr int kHalo = 256;                    // LDS halo on each side of a group
+constexpr int kMaxSinglePassPat = kHalo;      // chunked kernels take L <= 256
 constexpr int kMaxTablePat = 16;              // carry tables (self-overlapping patterns) take L <= 16
 constexpr int kStage = kTile + kTile / 8 + 64; // LDS stage for a tile's output (bytes)
 
-// ---- decoupled look-back state words ----------------------------------------
-// One 64-bit word per published value, written and read with relaxed agent-scope
-// atomics, so flag and payload travel together:
-//   [63:48] epoch  [47:46] status  [45:0] payload
-// A word whose epoch is not the launch's epoch reads as "not ready", so the state
-// array never needs clearing between launches (it is cleared when the 16-bit epoch
-// wraps).
-constexpr uint64_t kStatusAgg = 1;   // tile aggregate published
-constexpr uint64_t kStatusInc = 2;   // inclusive prefix published
+// A group is kGroupTiles consecutive tiles handled by one workgroup; a chunk is
+// a run of consecutive groups handled by one count, one spine and one emit launch.
+constexpr int kGroupTiles = 2;
+constexpr int kGroupBytes = kGroupTiles * kTile;    // 16 KiB of source per workgroup
+
+// Stats words carry the apply's 16-bit epoch in their top bits, so a stale word
+// from an earlier apply is told apart from this one's.
 constexpr int kEpochShift = 48;
-constexpr int kStatusShift = 46;
-constexpr uint64_t kPayloadMask = (1ull << 46) - 1;
-// table entries: payload = carry(6 bits) << 40 | count(40 bits)
+// 64-bit carry tables used while composing group tables: carry << 40 | count
 constexpr int kCarryShift = 40;
 constexpr uint64_t kCountMask = (1ull << 40) - 1;
 
@@ -47,17 +43,25 @@ struct TileInfo {
     int32_t mode;       // apply mode that wrote the record
 };
 
+// Kept state at a group or chunk boundary (the spine writes one per group and one
+// after each chunk).
+struct ChunkState {
+    int64_t k_base;     // kept starts before the chunk
+    int32_t carry;      // leading bytes of the chunk covered by an earlier kept match
+    int32_t pad;
+};
+
 enum Mode : int32_t {
     kModeBorderFree = 1,   // pattern cannot overlap itself: every candidate >= F is kept
-    kModeTable = 2,        // self-overlapping, L <= 16: carry-state tables + look-back
+    kModeTable = 2,        // self-overlapping, L <= 16: per-carry-state tables
     kModeBitmap = 3,       // general: candidate bitmap, serial greedy, scatter
 };
 
-// stats words written by a launch
+// stats words written by a pass
 struct LaunchStats {
     uint64_t k_total;      // epoch-tagged total kept count
     uint64_t last_kept;    // epoch-tagged (last kept + 1), written by the bitmap greedy
-    uint64_t pad[2];       // pad[1]: epoch-tagged abort flag of a look-back launch
+    uint64_t pad[2];
 };
 
 struct ReplaceParams {
@@ -72,21 +76,28 @@ struct ReplaceParams {
     int32_t r;
     uint32_t pat4;         // first min(L,4) pattern bytes, little-endian
     uint32_t mask4;        // byte mask of those bytes
-    uint64_t *state;       // look-back words
+    uint32_t *gsum;        // per-group kept count, or LT table entries per group
+    ChunkState *chunks;    // num_chunks + 1 chunk states
+    ChunkState *chunks_pre; // per group: kept starts before it, carry into it
     TileInfo *info;        // per-tile records (num_tiles)
     uint32_t *bitmap;      // candidate / kept bitmap (bitmap mode)
     LaunchStats *stats;
     uint32_t epoch;        // 1..65535
     int32_t write_output;  // 0: count only
     int64_t num_tiles;
+    int64_t num_groups;
 };
 
 // ---- launchers (cr_kernels.hip) --------------------------------------------
-hipError_t launch_border_free(const ReplaceParams &p, hipStream_t s);
-hipError_t launch_table(const ReplaceParams &p, int table_size, hipStream_t s);
+// One chunk of groups [first_group, first_group + groups): count, spine, emit.
+hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int64_t first_group,
+                        int64_t groups, hipStream_t s);
+hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_t first_group,
+                        int64_t groups, int64_t chunk, int last_chunk, hipStream_t s);
+hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int64_t first_group,
+                       int64_t groups, hipStream_t s);
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s);
-hipError_t launch_bitmap_scatter(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_empty_pattern(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *ptr, uint64_t value, size_t count, hipStream_t s);
 hipError_t launch_query(const ReplaceParams &p, int mode, int64_t limit,

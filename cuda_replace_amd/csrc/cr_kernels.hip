@@ -1,23 +1,31 @@
 // cr_kernels.hip -- HIP kernels for leftmost non-overlapping byte replace on gfx950.
 //
-// One workgroup (256 threads, 4 wave64) processes one 8 KiB tile of the source in a
-// single pass: stage tile + halo in LDS, build 32-bit candidate masks per thread,
-// resolve the leftmost-greedy suppression, obtain the tile's output offset with a
-// decoupled look-back over earlier tiles, then assemble the tile's contiguous output
-// in LDS and write it with aligned 16-byte stores.
+// The source is cut into 8 KiB tiles; kGroupTiles consecutive tiles form a group,
+// handled by one 256-thread workgroup (4 wave64, 32 source bytes per thread); runs of
+// consecutive groups form chunks (by default one chunk holds the whole pass).  Each
+// chunk is processed by three launches on the session stream (reduce-then-scan):
+//   count  stages the group in LDS (global_load_lds, 16 B per lane), builds each
+//          thread's 32-position candidate mask with byte-aligned dword compares, and
+//          writes the group's summary: its kept count for a border-free pattern, its
+//          carry table (carry in -> carry out, kept) for a self-overlapping one;
+//   spine  one workgroup scans the chunk's summaries into each group's exclusive
+//          state (kept starts before it, carry into it);
+//   emit   stages the group again, resolves the kept starts of each tile from the
+//          group's state, assembles the tile's output in LDS and writes it with
+//          16-byte stores.
+// No workgroup ever waits on another: launch boundaries order the chunks.
 //
 // Replaces the reference's mark / suppress / scan / scatter chain
-// (cuda_replace.cu:737-866 fused mark+suppress, 237-334 three-stage scan,
-// 1391-1608 shrink-pad / expansion scatter, 2097-2159 apply_once) with one kernel.
+// (cuda_replace.cu:737-866 fused mark+suppress, 905-968 serial greedy, 237-334 three-stage
+// scan, 1391-1608 shrink-pad / expansion scatter, 2097-2159 apply_once).
 //
 // Suppression modes:
-//   border-free pattern (no proper prefix equals a suffix): occurrences can never
-//     overlap, so every candidate at or above the seed floor is kept; the carry into a
-//     tile is recomputed from the L-1 bytes left of it; look-back sums kept counts.
-//   self-overlapping pattern, L <= 16: each thread summarises its 32 bytes as a table
-//     carry_in -> (carry_out, kept) over the L carry states; tables compose
-//     associatively (wave64 shuffle scan, then look-back over tile tables).
-//   otherwise: candidate bitmap, one-wave serial greedy, then the same scatter.
+//   border-free pattern (no proper prefix equals a suffix): occurrences never overlap,
+//     so every candidate at or above the seed floor is kept; the carry into a group is
+//     recomputed from the L-1 bytes left of it.
+//   self-overlapping pattern, L <= 16: per-thread tables over the L carry states,
+//     composed in position order (wave64 shuffle scans, then across groups).
+//   otherwise: candidate bitmap, one-wave serial greedy, then the emit kernel scatters.
 #include "cr_internal.h"
 
 #include <limits.h>
@@ -29,41 +37,8 @@ namespace cr {
 constexpr int kWave = 64;
 constexpr int kWaves = kThreads / kWave;
 constexpr int kNegInf = INT_MIN / 4;
-// Bound on look-back polls of one tile (legitimate waits are a few microseconds).
-constexpr int kSpinLimit = 1 << 18;
 // Replacements up to this length are staged in LDS; longer ones are read from HBM.
 constexpr int kRepLds = 256;
-
-// A look-back that waits too long raises the launch's abort flag; every other
-// waiter sees it and leaves, and the host re-runs the pass on the bitmap path.
-__device__ __forceinline__ bool aborted(const ReplaceParams &p)
-{
-    const uint64_t w = __hip_atomic_load(&p.stats->pad[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return (uint32_t)(w >> kEpochShift) == p.epoch;
-}
-__device__ __forceinline__ void raise_abort(const ReplaceParams &p)
-{
-    __hip_atomic_store(&p.stats->pad[1], ((uint64_t)p.epoch << kEpochShift) | 1u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---- look-back words --------------------------------------------------------
-__device__ __forceinline__ uint64_t make_word(uint32_t epoch, uint64_t status, uint64_t payload)
-{
-    return ((uint64_t)epoch << kEpochShift) | (status << kStatusShift) | (payload & kPayloadMask);
-}
-__device__ __forceinline__ uint64_t load_word(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_word(uint64_t *p, uint64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t word_status(uint64_t w, uint32_t epoch)
-{
-    return ((uint32_t)(w >> kEpochShift) == epoch) ? (uint32_t)((w >> kStatusShift) & 3u) : 0u;
-}
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
@@ -126,7 +101,7 @@ __device__ __forceinline__ Table<LT> tidentity()
     return r;
 }
 
-// ---- tile kernel ------------------------------------------------------------
+// ---- LDS / HBM copy helpers ---------------------------------------------------
 // 16 bytes starting at byte offset `off` of an LDS buffer (4-byte aligned base):
 // five aligned dword reads and a byte align.
 __device__ __forceinline__ uint4 lds_read16(const uint8_t *base, int off)
@@ -235,53 +210,7 @@ __device__ __forceinline__ void store_from_global(uint8_t *dst, const uint8_t *s
         dst[i] = src[i];
 }
 
-// Sum over the wave of the published words at addr[0..S) of every lane (null
-// addresses contribute 0), waiting until every one is published.  All loads of a
-// round are in flight together.  Returns false when the wait gave up (abort).
-// Second half of a poll whose words w were already loaded from addr.
-template <int S>
-__device__ __forceinline__ bool finish_poll(const ReplaceParams &p, const uint64_t *const (&addr)[S], uint64_t (&w)[S],
-                                            int64_t *sum)
-{
-    int spins = 0;
-    while (true) {
-        bool nr = false;
-#pragma unroll
-        for (int q = 0; q < S; ++q)
-            nr |= word_status(w[q], p.epoch) == 0;
-        if (!__ballot(nr))
-            break;
-        if (++spins > kSpinLimit || aborted(p))
-            return false;
-        __builtin_amdgcn_s_sleep(2);
-#pragma unroll
-        for (int q = 0; q < S; ++q)
-            if (word_status(w[q], p.epoch) == 0)
-                w[q] = load_word(addr[q]);
-    }
-    int64_t v = 0;
-#pragma unroll
-    for (int q = 0; q < S; ++q)
-        v += (int64_t)(w[q] & kPayloadMask);
-#pragma unroll
-    for (int dd = kWave / 2; dd > 0; dd >>= 1)
-        v += __shfl_xor(v, dd, kWave);
-    *sum = v;
-    return true;
-}
-
-template <int S>
-__device__ __forceinline__ bool poll_sum(const ReplaceParams &p, const uint64_t *const (&addr)[S], int64_t *sum)
-{
-    uint64_t w[S];
-#pragma unroll
-    for (int q = 0; q < S; ++q)
-        w[q] = addr[q] ? load_word(addr[q]) : make_word(p.epoch, kStatusAgg, 0);
-    return finish_poll<S>(p, addr, w, sum);
-}
-
-
-// 64-bit carry tables for the prefix hierarchy: entry = carry_out << 40 | count.
+// 64-bit carry tables for composing many group tables: entry = carry_out << 40 | count.
 template <int LT>
 struct Table64 {
     uint64_t e[LT];
@@ -357,106 +286,386 @@ __device__ __forceinline__ Table64<LT> wave_broadcast0(const Table64<LT> &t)
     return r;
 }
 
-// Loads the published LT-word table at addr (identity when addr is null), waiting
-// until every word is published.  Returns false when the wait gave up.
+// Ordered reduction of 32-bit carry tables across the wave: lane 0 ends with
+// lane0 ∘ lane1 ∘ ... ∘ lane63.
 template <int LT>
-__device__ __forceinline__ bool poll_table(const ReplaceParams &p, const uint64_t *addr, Table64<LT> &t)
+__device__ __forceinline__ Table<LT> wave_reduce_table(Table<LT> t, int lane)
 {
-    uint64_t w[LT];
 #pragma unroll
-    for (int c = 0; c < LT; ++c)
-        w[c] = addr ? load_word(addr + c) : make_word(p.epoch, kStatusAgg, (uint64_t)c << kCarryShift);
-    int spins = 0;
-    while (true) {
-        bool nr = false;
+    for (int d = 1; d < kWave; d <<= 1) {
+        Table<LT> o;
 #pragma unroll
         for (int c = 0; c < LT; ++c)
-            nr |= word_status(w[c], p.epoch) == 0;
-        if (!__ballot(nr))
-            break;
-        if (++spins > kSpinLimit || aborted(p))
-            return false;
-        __builtin_amdgcn_s_sleep(2);
-#pragma unroll
-        for (int c = 0; c < LT; ++c)
-            if (word_status(w[c], p.epoch) == 0)
-                w[c] = load_word(addr + c);
+            o.e[c] = __shfl_down(t.e[c], d, kWave);
+        if ((lane & (2 * d - 1)) == 0 && lane + d < kWave)
+            t = tcompose<LT>(t, o);
     }
-#pragma unroll
-    for (int c = 0; c < LT; ++c)
-        t.e[c] = w[c] & kPayloadMask;
-    return true;
+    return t;
 }
 
-// ---- persistent tile kernel -------------------------------------------------
-// The next tile is fetched straight into LDS with global_load_lds (no registers):
-// waves 2-3 load the 8 KiB body (1 KiB per instruction), wave 1 the two 256-byte
-// halos (4 bytes per lane).  Wave 0 never issues these loads, so its look-back
-// polls never wait behind them.
-__device__ __forceinline__ void prefetch_lds(const ReplaceParams &p, int64_t tile, int tid, bool halos,
-                                             uint8_t *s_in)
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1)
+        v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// ---- group staging ----------------------------------------------------------
+// LDS image of a group: [kHalo bytes before | kGroupBytes | kHalo bytes after].
+constexpr int kInBytes = kHalo + kGroupBytes + kHalo;
+
+// Issues the group's loads straight into LDS with global_load_lds (no registers):
+// every wave streams 1 KiB per instruction of the body; wave 0 fetches the left
+// halo and wave 1 the right halo (4 bytes per lane).  The caller waits with
+// s_waitcnt + a barrier.  The source is a 16-byte aligned device buffer, so the
+// granule that holds the last valid byte is readable; bytes at or past n are never
+// matched or emitted.
+__device__ __forceinline__ void load_group(const ReplaceParams &p, int64_t t0, int tid, bool left_halo,
+                                           uint8_t *s_in)
 {
     typedef __attribute__((address_space(3))) void lds_void;
-    const int64_t start = tile * kTile;
+    const int64_t start = t0 * kTile;
+    const int64_t avail = p.n - start;
     const int wave = tid / kWave;
     const int lane = tid & (kWave - 1);
-    if (wave >= 2) {
-        // The source is a 16-byte aligned device allocation (16-byte granular), so
-        // the granule holding the last valid byte is readable; bytes past n are
-        // never matched or emitted.
-        const int64_t avail = p.n - start;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int off = (wave - 2) * 1024 + k * 2048;
-            if (off + lane * 16 < avail)
-                __builtin_amdgcn_global_load_lds((const void *)(p.src + start + off + lane * 16),
-                                                 (lds_void *)(s_in + kHalo + off), 16, 0, 0);
+    for (int k = 0; k < kGroupBytes / (kWaves * 1024); ++k) {
+        const int off = k * (kWaves * 1024) + wave * 1024;
+        if (off + lane * 16 < avail)
+            __builtin_amdgcn_global_load_lds((const void *)(p.src + start + off + lane * 16),
+                                             (lds_void *)(s_in + kHalo + off), 16, 0, 0);
+    }
+    if (wave == 0 && left_halo && start >= kHalo)
+        __builtin_amdgcn_global_load_lds((const void *)(p.src + start - kHalo + lane * 4), (lds_void *)(s_in), 4,
+                                         0, 0);
+    if (wave == 1 && start + kGroupBytes + lane * 4 < p.n)
+        __builtin_amdgcn_global_load_lds((const void *)(p.src + start + kGroupBytes + lane * 4),
+                                         (lds_void *)(s_in + kHalo + kGroupBytes), 4, 0, 0);
+}
+
+// Candidate mask of the 32 positions [gseg, gseg+32) whose bytes start at
+// s_tile + seg in LDS: bit j set iff the pattern occurs at gseg + j and
+// floor <= gseg + j <= n - L.  Reads up to L - 1 bytes past the segment.
+__device__ __forceinline__ uint32_t cand_mask(const ReplaceParams &p, const uint8_t *s_tile, const uint8_t *s_pat,
+                                              int seg, int64_t gseg)
+{
+    const int L = p.L;
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(s_tile + seg);
+    uint32_t d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        d[k] = w32[k];
+    const uint32_t pat4 = p.pat4, mask4 = p.mask4;
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 31; j >= 0; --j) {
+        const uint32_t w = __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
+        m = (m << 1) | (uint32_t)(((w ^ pat4) & mask4) == 0u);
+    }
+    const int64_t lo = p.floor - gseg;
+    const int64_t hi = p.n - (int64_t)L - gseg;
+    uint32_t rm = 0;
+    if (hi >= 0 && lo <= 31 && lo <= hi) {
+        const int a = lo < 0 ? 0 : (int)lo;
+        const int b = hi > 31 ? 31 : (int)hi;
+        rm = (b == 31 ? 0xffffffffu : ((1u << (b + 1)) - 1u)) & (0xffffffffu << a);
+    }
+    m &= rm;
+    if (L > 4 && m) {
+        uint32_t mm = m;
+        m = 0;
+        const uint8_t *sp = s_tile + seg;
+        while (mm) {
+            const int j = __builtin_ctz(mm);
+            mm &= mm - 1;
+            bool ok = true;
+            for (int k = 4; k < L; ++k) {
+                if (sp[j + k] != s_pat[k]) {
+                    ok = false;
+                    break;
+                }
+            }
+            m |= (uint32_t)ok << j;
         }
-    } else if (wave == 1 && halos) {
-        // left halo [start-256, start), right halo [start+kTile, start+kTile+256)
-        if (start >= kHalo)
-            __builtin_amdgcn_global_load_lds((const void *)(p.src + start - kHalo + lane * 4),
-                                             (lds_void *)(s_in), 4, 0, 0);
-        if (start + kTile + lane * 4 < p.n)
-            __builtin_amdgcn_global_load_lds((const void *)(p.src + start + kTile + lane * 4),
-                                             (lds_void *)(s_in + kHalo + kTile), 4, 0, 0);
+    }
+    return m;
+}
+
+// Per-thread carry table over carry-in c in [0, LT): the greedy over 32 candidate
+// positions with the first c blocked gives (carry into the next segment, kept).
+template <int LT>
+__device__ __forceinline__ Table<LT> segment_table(uint32_t m, int L)
+{
+    Table<LT> t;
+#pragma unroll
+    for (int c = 0; c < LT; ++c) {
+        int lst;
+        const uint32_t km = greedy32(m, c, L, &lst);
+        int co = lst + L - 32;
+        co = co < 0 ? 0 : co;
+        t.e[c] = ((uint32_t)co << 24) | (uint32_t)__builtin_popcount(km);
+    }
+    return t;
+}
+
+// ---- count: one summary per group ------------------------------------------
+// Border-free pattern: every candidate at or above the floor is kept, so the
+// group's kept count is a popcount.  Self-overlapping pattern: the group's carry
+// table (carry in -> carry out, kept), composed tile by tile in position order.
+// Workgroups never wait on each other.
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kThreads)
+count_kernel(ReplaceParams p, int64_t first_group)
+{
+    __shared__ __align__(16) uint8_t s_in[kInBytes];
+    __shared__ uint8_t s_pat[kMaxSinglePassPat];
+    __shared__ uint32_t s_red[kWaves];
+    __shared__ uint32_t s_wt[kWaves][LT];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int64_t g = first_group + blockIdx.x;
+    const int64_t t0 = g * kGroupTiles;
+    if (t0 >= p.num_tiles)
+        return;
+    for (int i = tid; i < p.L; i += kThreads)
+        s_pat[i] = p.pat[i];
+    load_group(p, t0, tid, false, s_in);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    const int seg = tid * kBytesPerThread;
+    if constexpr (MODE == kModeBorderFree) {
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int i = 0; i < kGroupTiles; ++i) {
+            const int64_t ti = t0 + i;
+            if (ti < p.num_tiles)
+                cnt += __builtin_popcount(cand_mask(p, s_in + kHalo + i * kTile, s_pat, seg, ti * kTile + seg));
+        }
+        cnt = wave_sum_u32(cnt);
+        if (lane == 0)
+            s_red[wave] = cnt;
+        __syncthreads();
+        if (tid == 0)
+            p.gsum[g] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    } else {
+        Table<LT> grp = tidentity<LT>();
+        for (int i = 0; i < kGroupTiles; ++i) {
+            const int64_t ti = t0 + i;
+            if (ti >= p.num_tiles)
+                break;
+            const uint32_t m = cand_mask(p, s_in + kHalo + i * kTile, s_pat, seg, ti * kTile + seg);
+            Table<LT> t = wave_reduce_table<LT>(segment_table<LT>(m, p.L), lane);
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    s_wt[wave][c] = t.e[c];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                Table<LT> wt;
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    wt.e[c] = s_wt[w][c];
+                grp = tcompose<LT>(grp, wt);
+            }
+            __syncthreads();
+        }
+        if (tid < LT)
+            p.gsum[g * LT + tid] = tsel<LT>(grp, (uint32_t)tid);
     }
 }
 
-// Persistent: gridDim.x workgroups, all co-resident (cooperative launch), walk the
-// tiles blockIdx.x, blockIdx.x + gridDim.x, ...  Every tile a look-back waits on
-// belongs to a resident workgroup and an earlier or the same round, so the waits
-// always end; the bounded spin only turns an unexpected stall into an abort that
-// the host answers by re-running the pass on the bitmap path.
+// ---- spine: exclusive kept state of every group of a chunk -------------------
+// One workgroup of kSpineThreads walks the chunk's group summaries in passes of
+// kSpineThreads * V groups (V = kSpineWords / words per summary, so a pass stages
+// 64 KiB): a coalesced load into LDS, each thread folds its V consecutive groups,
+// a block-wide ordered scan of the thread folds, then each thread writes the
+// exclusive state (kept before the group, carry into it) of its groups.  The state after the chunk goes to chunks[chunk + 1]; the last
+// chunk also publishes the pass's kept total.
+constexpr int kSpineThreads = 1024;
+constexpr int kSpineWaves = kSpineThreads / kWave;
+constexpr int kSpineWords = 16;
+
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kSpineThreads)
+spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk, int last_chunk)
+{
+    constexpr int W = MODE == kModeTable ? LT : 1;   // words per group summary
+    constexpr int V = kSpineWords / W;                // groups per thread and pass
+    constexpr int kPass = kSpineThreads * V;
+    __shared__ uint32_t s_v[kPass * W + kSpineThreads];   // padded: no bank conflicts on the fold
+    __shared__ uint64_t s_wt[kSpineWaves][LT];
+    __shared__ int64_t s_wsum[kSpineWaves];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    int64_t k = 0;    // kept before the current pass
+    int carry = 0;    // carry into the current pass
+    if (chunk > 0) {
+        k = p.chunks[chunk].k_base;
+        carry = p.chunks[chunk].carry;
+    }
+    // LDS slot of word w of value i: one pad word per thread's V values, so the
+    // threads' folds read with an odd stride (no bank conflicts)
+    auto slot = [](int i, int w) { return (i * W + w) + (i / V); };
+
+    for (int64_t base = 0; base < groups; base += kPass) {
+        const int cnt = (int)(groups - base < kPass ? groups - base : kPass);
+        for (int i = tid; i < cnt * W; i += kSpineThreads)
+            s_v[slot(i / W, i % W)] = p.gsum[(first_group + base) * W + i];
+        __syncthreads();
+        const int v0 = tid * V;
+        if constexpr (MODE != kModeTable) {
+            int64_t mine = 0;
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+                if (v0 + j < cnt)
+                    mine += s_v[slot(v0 + j, 0)];
+            // block exclusive scan of the thread sums
+            int64_t inc = mine;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int64_t u = __shfl_up(inc, d, kWave);
+                if (lane >= d)
+                    inc += u;
+            }
+            if (lane == kWave - 1)
+                s_wsum[wave] = inc;
+            __syncthreads();
+            int64_t before = 0, total = 0;
+            for (int w = 0; w < kSpineWaves; ++w) {
+                if (w < wave)
+                    before += s_wsum[w];
+                total += s_wsum[w];
+            }
+            int64_t run = k + before + inc - mine;
+            for (int j = 0; j < V; ++j) {
+                if (v0 + j < cnt) {
+                    ChunkState cs;
+                    cs.k_base = run;
+                    cs.carry = 0;
+                    cs.pad = 0;
+                    p.chunks_pre[first_group + base + v0 + j] = cs;
+                    run += s_v[slot(v0 + j, 0)];
+                }
+            }
+            k += total;
+        } else {
+            // fold my groups' tables, then an ordered block scan of the folds
+            Table64<LT> mine = tidentity64<LT>();
+            for (int j = 0; j < V; ++j) {
+                if (v0 + j < cnt) {
+                    Table<LT> t;
+#pragma unroll
+                    for (int c = 0; c < LT; ++c)
+                        t.e[c] = s_v[slot(v0 + j, c)];
+                    mine = tcompose64<LT>(mine, widen<LT>(t));
+                }
+            }
+            Table64<LT> inc = mine;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                Table64<LT> nb;
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    nb.e[c] = __shfl_up(inc.e[c], d, kWave);
+                if (lane >= d)
+                    inc = tcompose64<LT>(nb, inc);
+            }
+            Table64<LT> ex;
+#pragma unroll
+            for (int c = 0; c < LT; ++c) {
+                const uint64_t v = __shfl_up(inc.e[c], 1, kWave);
+                ex.e[c] = lane == 0 ? ((uint64_t)c << kCarryShift) : v;
+            }
+            if (lane == kWave - 1) {
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    s_wt[wave][c] = inc.e[c];
+            }
+            __syncthreads();
+            Table64<LT> wpre = tidentity64<LT>(), tot = tidentity64<LT>();
+            for (int w = 0; w < kSpineWaves; ++w) {
+                Table64<LT> wt;
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    wt.e[c] = s_wt[w][c];
+                if (w < wave)
+                    wpre = tcompose64<LT>(wpre, wt);
+                tot = tcompose64<LT>(tot, wt);
+            }
+            const uint64_t e = tsel64<LT>(tcompose64<LT>(wpre, ex), (uint32_t)carry);
+            int64_t rk = k + (int64_t)(e & kCountMask);
+            int rc = (int)(e >> kCarryShift);
+            for (int j = 0; j < V; ++j) {
+                if (v0 + j < cnt) {
+                    ChunkState cs;
+                    cs.k_base = rk;
+                    cs.carry = rc;
+                    cs.pad = 0;
+                    p.chunks_pre[first_group + base + v0 + j] = cs;
+                    Table<LT> t;
+#pragma unroll
+                    for (int c = 0; c < LT; ++c)
+                        t.e[c] = s_v[slot(v0 + j, c)];
+                    const uint32_t te = tsel<LT>(t, (uint32_t)rc);
+                    rk += (int64_t)(te & 0xffffffu);
+                    rc = (int)(te >> 24);
+                }
+            }
+            const uint64_t te = tsel64<LT>(tot, (uint32_t)carry);
+            k += (int64_t)(te & kCountMask);
+            carry = (int)(te >> kCarryShift);
+        }
+        __syncthreads();   // s_v and the wave slots are reused by the next pass
+    }
+    if (tid == 0) {
+        ChunkState cs;
+        cs.k_base = k;
+        cs.carry = carry;
+        cs.pad = 0;
+        p.chunks[chunk + 1] = cs;
+        if (last_chunk)
+            p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | (uint64_t)k;
+    }
+}
+
+// ---- emit: resolve kept starts and write each tile's output ----------------
+// The group's kept prefix and carry come from the spine's record for the group.
+// The bitmap mode instead reads each tile's prefix and carry from the records of
+// the serial greedy, and the kept starts from the bitmap.
 template <int MODE, int LT>
 __global__ void __launch_bounds__(kThreads)
-tile_kernel(ReplaceParams p)
+emit_kernel(ReplaceParams p, int64_t first_group)
 {
-    __shared__ __align__(16) uint8_t s_in[kHalo + kTile + kHalo];
+    __shared__ __align__(16) uint8_t s_in[kInBytes];
     __shared__ __align__(16) uint8_t s_out[kStage];
     __shared__ uint8_t s_pat[kMaxSinglePassPat];
     __shared__ __align__(16) uint8_t s_rep[kRepLds + 32];
     __shared__ int s_wsum[kWaves];
     __shared__ int s_wmax[kWaves];
     __shared__ int s_halo_last;
-    __shared__ int s_abort;
-    __shared__ int64_t s_kexcl;
-    __shared__ int s_carry_in;
-    __shared__ uint32_t s_wtab[kWaves][LT > 1 ? LT : 1];
+    __shared__ uint32_t s_wtab[kWaves][LT];
 
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
     const int L = p.L;
     const int r = p.r;
     const int64_t n = p.n;
     const int64_t nt = p.num_tiles;
-    const int64_t G = gridDim.x;
     const int64_t diff = (int64_t)r - (int64_t)L;
-    const bool need_src = MODE != kModeBitmap || p.write_output;
-    const bool halos = MODE != kModeBitmap;
-    int tid = threadIdx.x;
-
-    int64_t tile = (int64_t)blockIdx.x;
-    if (tile >= nt)
+    const int64_t g = first_group + blockIdx.x;
+    const int64_t t0 = g * kGroupTiles;
+    if (t0 >= nt)
         return;
+
     if (MODE != kModeBitmap) {
         for (int i = tid; i < L; i += kThreads)
             s_pat[i] = p.pat[i];
@@ -466,110 +675,77 @@ tile_kernel(ReplaceParams p)
             s_rep[i] = p.rep[i];
     }
     if (tid == 0)
-        s_abort = 0;
-    if (need_src)
-        prefetch_lds(p, tile, tid, halos, s_in);
+        s_halo_last = kNegInf;
 
-    for (; tile < nt; tile += G) {
-        // Re-derive the thread index each round through an opaque register move:
-        // keeps lane-dependent addresses from being hoisted out of the loop (they
-        // would pin dozens of VGPRs for the whole kernel).
-        asm volatile("" : "+v"(tid));
-        const int lane = tid & (kWave - 1);
-        const int wave = tid / kWave;
-        const int seg = tid * kBytesPerThread;     // tile-relative start of my segment
-        const int64_t start = tile * kTile;
+    // ---- kept starts before the group and carry into it (from the spine) --------
+    load_group(p, t0, tid, MODE == kModeBorderFree, s_in);
+    int64_t k_run = 0;   // kept starts before the current tile
+    int c_run = 0;       // carry into the current tile
+    if (MODE != kModeBitmap) {
+        const ChunkState pre = p.chunks_pre[g];
+        k_run = pre.k_base;
+        c_run = pre.carry;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    const int seg = tid * kBytesPerThread;
+    for (int i = 0; i < kGroupTiles; ++i) {
+        const int64_t ti = t0 + i;
+        if (ti >= nt)
+            break;
+        const int64_t start = ti * kTile;
         const int valid = (int)((n - start) < kTile ? (n - start) : kTile);
-        const bool last_tile = tile == nt - 1;
+        const bool last_tile = ti == nt - 1;
         const int seg_valid = valid - seg;
         const int jend = seg_valid < 32 ? (seg_valid > 0 ? seg_valid : 0) : 32;
+        const int in_off = kHalo + i * kTile;   // LDS offset of the tile's first byte
 
-        if (wave >= 1)
-            __builtin_amdgcn_s_waitcnt(0);         // this wave's prefetch has landed
-        if (tid == 0)
-            s_halo_last = kNegInf;
-        uint32_t m = 0;   // candidate mask, then kept mask, of this thread's 32 positions
-        if constexpr (MODE == kModeBitmap)
-            m = p.bitmap[tile * (kTile / 32) + tid];
-        __syncthreads();
-
-        // ---- candidate mask ----------------------------------------------------
-        if constexpr (MODE != kModeBitmap) {
-            const int64_t gseg = start + seg;
-            const uint32_t *w32 = reinterpret_cast<const uint32_t *>(s_in + kHalo + seg);
-            uint32_t d[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                d[k] = w32[k];
-            const uint32_t pat4 = p.pat4, mask4 = p.mask4;
-#pragma unroll
-            for (int j = 31; j >= 0; --j) {
-                const uint32_t w = __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
-                m = (m << 1) | (uint32_t)(((w ^ pat4) & mask4) == 0u);
-            }
-            // bounds: floor <= pos <= n - L
-            const int64_t lo = p.floor - gseg;
-            const int64_t hi = n - (int64_t)L - gseg;
-            uint32_t rm = 0;
-            if (hi >= 0 && lo <= 31 && lo <= hi) {
-                const int a = lo < 0 ? 0 : (int)lo;
-                const int b = hi > 31 ? 31 : (int)hi;
-                rm = (b == 31 ? 0xffffffffu : ((1u << (b + 1)) - 1u)) & (0xffffffffu << a);
-            }
-            m &= rm;
-            if (L > 4 && m) {
-                uint32_t mm = m;
-                m = 0;
-                const uint8_t *sp = s_in + kHalo + seg;
-                while (mm) {
-                    const int j = __builtin_ctz(mm);
-                    mm &= mm - 1;
-                    bool ok = true;
-                    for (int k = 4; k < L; ++k) {
-                        if (sp[j + k] != s_pat[k]) {
-                            ok = false;
-                            break;
-                        }
-                    }
-                    m |= (uint32_t)ok << j;
-                }
-            }
+        uint32_t m = 0;
+        if constexpr (MODE == kModeBitmap) {
+            m = p.bitmap[ti * (kTile / 32) + tid];
+            k_run = p.info[ti].k_excl;
+            c_run = p.info[ti].carry_in;
+        } else {
+            m = cand_mask(p, s_in + in_off, s_pat, seg, start + seg);
         }
-
-        // ---- border-free: carry from candidates in the L-1 bytes left of the tile
         if constexpr (MODE == kModeBorderFree) {
-            if (tile > 0 && tid < L - 1) {
-                const int q = -1 - tid;                      // tile-relative position
-                const int64_t g = start + q;
-                if (g >= p.floor && g + L <= n) {
-                    const uint8_t *sp = s_in + kHalo + q;
-                    bool ok = true;
-                    for (int k = 0; k < L; ++k) {
-                        if (sp[k] != s_pat[k]) {
-                            ok = false;
-                            break;
+            if (i == 0) {
+                // carry from a candidate in the L-1 bytes left of the group; it is kept
+                // (border-free occurrences never overlap) unless below the floor
+                if (ti > 0 && tid < L - 1) {
+                    const int q = -1 - tid;
+                    const int64_t gp = start + q;
+                    if (gp >= p.floor && gp + L <= n) {
+                        const uint8_t *sp = s_in + in_off + q;
+                        bool ok = true;
+                        for (int k = 0; k < L; ++k) {
+                            if (sp[k] != s_pat[k]) {
+                                ok = false;
+                                break;
+                            }
                         }
+                        if (ok)
+                            atomicMax(&s_halo_last, q);
                     }
-                    if (ok)
-                        atomicMax(&s_halo_last, q);
                 }
+                __syncthreads();
+                c_run = s_halo_last + L > 0 ? s_halo_last + L : 0;
             }
-            __syncthreads();
         }
+        const int c_t = c_run;
 
-        int c_i = 0;        // leading bytes of my segment covered by an earlier kept match
-        int k_loc = 0;      // kept starts in the tile before my segment
-        int tile_k = 0;     // kept starts in the tile
-        int tile_last = kNegInf;   // last kept start in the tile (tile-relative)
-        int c_t = 0;        // carry into the tile
-        int c_next = 0;     // carry into the next tile
-        int64_t k_excl = 0; // kept starts before the tile
+        int c_i = 0;          // leading bytes of my segment covered by an earlier kept match
+        int k_loc = 0;        // kept starts in the tile before my segment
+        int tile_k = 0;       // kept starts in the tile
+        int tile_last = kNegInf;
+        int c_next = 0;       // carry into the next tile
+        uint32_t kept = m;
 
         if constexpr (MODE == kModeBorderFree || MODE == kModeBitmap) {
-            // kept mask is final: scan counts (sum) and last kept (max)
-            const int k = __builtin_popcount(m);
+            const int kc = __builtin_popcount(m);
             const int lastq = m ? (seg + 31 - __builtin_clz(m)) : kNegInf;
-            int ks = k, lq = lastq;
+            int ks = kc, lq = lastq;
 #pragma unroll
             for (int dd = 1; dd < kWave; dd <<= 1) {
                 const int uk = __shfl_up(ks, dd, kWave);
@@ -587,16 +763,7 @@ tile_kernel(ReplaceParams p)
                 s_wmax[wave] = lq;
             }
             __syncthreads();
-            int init_last;
-            if constexpr (MODE == kModeBorderFree) {
-                init_last = s_halo_last;
-                c_t = init_last + L > 0 ? init_last + L : 0;
-            } else {
-                c_t = p.info[tile].carry_in;
-                k_excl = p.info[tile].k_excl;
-                init_last = c_t - L;   // a virtual kept start covering [.., c_t)
-            }
-            int wsum = 0, wmax = init_last;
+            int wsum = 0, wmax = c_t - L;   // a virtual kept start covering [.., c_t)
             for (int w = 0; w < kWaves; ++w) {
                 if (w < wave) {
                     wsum += s_wsum[w];
@@ -605,7 +772,7 @@ tile_kernel(ReplaceParams p)
                 tile_k += s_wsum[w];
                 tile_last = s_wmax[w] > tile_last ? s_wmax[w] : tile_last;
             }
-            k_loc = wsum + ks - k;
+            k_loc = wsum + ks - kc;
             const int prev_last = lq_ex > wmax ? lq_ex : wmax;
             c_i = prev_last + L - seg;
             if (c_i < 0)
@@ -615,149 +782,11 @@ tile_kernel(ReplaceParams p)
                 if (c_next < 0 || last_tile)
                     c_next = 0;
             } else {
-                c_next = last_tile ? 0 : p.info[tile + 1].carry_in;
+                c_next = last_tile ? 0 : p.info[ti + 1].carry_in;
             }
-        }
-
-        // Writes this thread's output (source runs between kept starts, the
-        // replacement at each kept start) into the tile-relative LDS stage.
-        auto stage_output = [&](uint32_t kept_mask) {
-            int o = (seg + c_i - c_t) + k_loc * (int)diff;
-            int j = c_i;
-            uint32_t M = j >= 32 ? 0u : (kept_mask & (0xffffffffu << j));
-            while (j < jend) {
-                const int q = M ? __builtin_ctz(M) : jend;
-                if (q > j) {
-                    lds_copy(s_out, o, s_in, kHalo + seg + j, q - j);
-                    o += q - j;
-                }
-                if (q >= jend)
-                    break;
-                lds_copy(s_out, o, s_rep, 0, r);
-                o += r;
-                j = q + L;
-                M = j >= 32 ? 0u : (M & (0xffffffffu << j));
-            }
-        };
-        // tile output length (before the look-back for border-free and bitmap)
-        int64_t out_len = (int64_t)valid + c_next - c_t + (int64_t)tile_k * diff;
-        bool staged = p.write_output && r <= kRepLds && out_len + 32 <= kStage;
-
-        uint32_t kept = m;
-        if constexpr (MODE == kModeBorderFree) {
-            // Hierarchical prefix of kept counts, fan-out 64: every tile publishes its
-            // count, the last tile of each 64-tile group also publishes the group sum,
-            // and likewise for groups of groups.  A tile's exclusive prefix is the sum
-            // of at most 63 words per level, all published by tiles of this or an
-            // earlier round shortly after their scans: no chain of inclusive prefixes.
-            uint64_t *const a0 = p.state;
-            uint64_t *const a1 = a0 + nt;
-            uint64_t *const a2 = a1 + (nt >> 6) + 1;
-            uint64_t *const a3 = a2 + (nt >> 12) + 1;
-            const int64_t d0 = tile & 63, d1 = (tile >> 6) & 63, d2 = (tile >> 12) & 63, d3 = tile >> 18;
-            const int64_t items = d0 + d1 + d2 + d3;
-            // prefix item it -> its published word: a0 words of this group, a1 of this
-            // 64-group block, a2 of this 4096-group block, a3 of all earlier blocks
-            auto item_addr = [&](int64_t it) -> const uint64_t * {
-                if (it < d0)
-                    return &a0[tile - d0 + it];
-                if (it < d0 + d1)
-                    return &a1[(tile >> 6) - d1 + (it - d0)];
-                if (it < d0 + d1 + d2)
-                    return &a2[(tile >> 12) - d2 + (it - d0 - d1)];
-                if (it < items)
-                    return &a3[it - d0 - d1 - d2];
-                return nullptr;
-            };
-            bool ok = true;
-            uint64_t pw[4];
-            if (wave == 0) {
-                if (lane == 0)
-                    store_word(&a0[tile], make_word(p.epoch, kStatusAgg, (uint64_t)tile_k));
-                if ((tile & 63) == 63) {
-                    int64_t s1 = 0;
-                    const uint64_t *g0[1] = {lane < 63 ? &a0[tile - 63 + lane] : nullptr};
-                    ok = poll_sum<1>(p, g0, &s1);
-                    s1 += tile_k;
-                    if (ok && lane == 0)
-                        store_word(&a1[tile >> 6], make_word(p.epoch, kStatusAgg, (uint64_t)s1));
-                    if (ok && ((tile >> 6) & 63) == 63) {
-                        const int64_t g = tile >> 6;
-                        int64_t s2 = 0;
-                        const uint64_t *g1[1] = {lane < 63 ? &a1[g - 63 + lane] : nullptr};
-                        ok = poll_sum<1>(p, g1, &s2);
-                        s2 += s1;
-                        if (ok && lane == 0)
-                            store_word(&a2[tile >> 12], make_word(p.epoch, kStatusAgg, (uint64_t)s2));
-                        if (ok && ((tile >> 12) & 63) == 63) {
-                            const int64_t h = tile >> 12;
-                            int64_t s3 = 0;
-                            const uint64_t *g2[1] = {lane < 63 ? &a2[h - 63 + lane] : nullptr};
-                            ok = poll_sum<1>(p, g2, &s3);
-                            s3 += s2;
-                            if (ok && lane == 0)
-                                store_word(&a3[tile >> 18], make_word(p.epoch, kStatusAgg, (uint64_t)s3));
-                        }
-                    }
-                }
-                // first poll round in flight while the output is staged
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint64_t *a = item_addr(q * kWave + lane);
-                    pw[q] = a ? load_word(a) : make_word(p.epoch, kStatusAgg, 0);
-                }
-            }
-            // build the output while the predecessors' words are on their way
-            if (staged)
-                stage_output(m);
-            if (wave == 0) {
-                int64_t acc = 0;
-                if (ok) {
-                    const uint64_t *pa[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        pa[q] = item_addr(q * kWave + lane);
-                    int64_t sum = 0;
-                    ok = finish_poll<4>(p, pa, pw, &sum);
-                    acc += sum;
-                }
-                for (int64_t base = 4 * kWave; ok && base < items; base += 4 * kWave) {
-                    const uint64_t *addr[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        addr[q] = item_addr(base + q * kWave + lane);
-                    int64_t sum = 0;
-                    ok = poll_sum<4>(p, addr, &sum);
-                    acc += sum;
-                }
-                if (lane == 0) {
-                    if (!ok) {
-                        raise_abort(p);
-                        s_abort = 1;
-                    }
-                    s_kexcl = acc;
-                }
-            }
-            __syncthreads();
-            if (s_abort)
-                return;
-            k_excl = s_kexcl;
-        } else if constexpr (MODE == kModeBitmap) {
-            if (staged)
-                stage_output(m);
-            __syncthreads();
         } else {
-            // per-thread table over carry states c in [0, LT)
-            Table<LT> t;
-#pragma unroll
-            for (int c = 0; c < LT; ++c) {
-                int lst;
-                const uint32_t km = greedy32(m, c, L, &lst);
-                int co = lst + L - 32;
-                co = co < 0 ? 0 : co;
-                t.e[c] = ((uint32_t)co << 24) | (uint32_t)__builtin_popcount(km);
-            }
-            // wave-inclusive scan of tables
+            // carry tables: wave-inclusive scan, then across the four waves
+            const Table<LT> t = segment_table<LT>(m, L);
             Table<LT> inc = t;
 #pragma unroll
             for (int dd = 1; dd < kWave; dd <<= 1) {
@@ -791,99 +820,13 @@ tile_kernel(ReplaceParams p)
                     wpre = tcompose<LT>(wpre, wt);
                 agg = tcompose<LT>(agg, wt);
             }
-            const Table<LT> thread_ex = tcompose<LT>(wpre, ex);
-
-            // hierarchical prefix of carry tables (same fan-out-64 layout as the
-            // border-free counts; entries composed in position order)
-            if (wave == 0) {
-                uint64_t *const t0 = p.state;
-                uint64_t *const t1 = t0 + nt * LT;
-                uint64_t *const t2 = t1 + ((nt >> 6) + 1) * LT;
-                uint64_t *const t3 = t2 + ((nt >> 12) + 1) * LT;
-                const Table64<LT> mine = widen<LT>(agg);
-                bool ok = true;
-#pragma unroll
-                for (int c = 0; c < LT; ++c)
-                    if (lane == c)
-                        store_word(&t0[tile * LT + c], make_word(p.epoch, kStatusAgg, mine.e[c]));
-                if ((tile & 63) == 63) {
-                    Table64<LT> gt;
-                    ok = poll_table<LT>(p, lane < 63 ? &t0[(tile - 63 + lane) * LT] : nullptr, gt);
-                    if (lane == 63)
-                        gt = mine;
-                    gt = wave_broadcast0<LT>(wave_compose<LT>(gt, lane));
-#pragma unroll
-                    for (int c = 0; c < LT; ++c)
-                        if (ok && lane == c)
-                            store_word(&t1[(tile >> 6) * LT + c], make_word(p.epoch, kStatusAgg, gt.e[c]));
-                    if (ok && ((tile >> 6) & 63) == 63) {
-                        const int64_t g = tile >> 6;
-                        Table64<LT> ht;
-                        ok = poll_table<LT>(p, lane < 63 ? &t1[(g - 63 + lane) * LT] : nullptr, ht);
-                        if (lane == 63)
-                            ht = gt;
-                        ht = wave_broadcast0<LT>(wave_compose<LT>(ht, lane));
-#pragma unroll
-                        for (int c = 0; c < LT; ++c)
-                            if (ok && lane == c)
-                                store_word(&t2[(tile >> 12) * LT + c], make_word(p.epoch, kStatusAgg, ht.e[c]));
-                        if (ok && ((tile >> 12) & 63) == 63) {
-                            const int64_t h = tile >> 12;
-                            Table64<LT> kt;
-                            ok = poll_table<LT>(p, lane < 63 ? &t2[(h - 63 + lane) * LT] : nullptr, kt);
-                            if (lane == 63)
-                                kt = ht;
-                            kt = wave_broadcast0<LT>(wave_compose<LT>(kt, lane));
-#pragma unroll
-                            for (int c = 0; c < LT; ++c)
-                                if (ok && lane == c)
-                                    store_word(&t3[(tile >> 18) * LT + c],
-                                               make_word(p.epoch, kStatusAgg, kt.e[c]));
-                        }
-                    }
-                }
-                // prefix items in position order: t3 blocks, t2, t1, t0
-                const int64_t d0 = tile & 63, d1 = (tile >> 6) & 63, d2 = (tile >> 12) & 63, d3 = tile >> 18;
-                const int64_t items = d0 + d1 + d2 + d3;
-                Table64<LT> pre = tidentity64<LT>();
-                for (int64_t base = 0; ok && base < items; base += kWave) {
-                    const int64_t it = base + lane;
-                    const uint64_t *addr = nullptr;
-                    if (it < d3)
-                        addr = &t3[it * LT];
-                    else if (it < d3 + d2)
-                        addr = &t2[((tile >> 12) - d2 + (it - d3)) * LT];
-                    else if (it < d3 + d2 + d1)
-                        addr = &t1[((tile >> 6) - d1 + (it - d3 - d2)) * LT];
-                    else if (it < items)
-                        addr = &t0[(tile - d0 + (it - d3 - d2 - d1)) * LT];
-                    Table64<LT> tt;
-                    ok = poll_table<LT>(p, addr, tt);
-                    pre = tcompose64<LT>(pre, wave_broadcast0<LT>(wave_compose<LT>(tt, lane)));
-                }
-                const uint64_t e0 = pre.e[0];   // carry 0 into tile 0
-                if (lane == 0) {
-                    if (!ok) {
-                        raise_abort(p);
-                        s_abort = 1;
-                    }
-                    s_kexcl = (int64_t)(e0 & kCountMask);
-                    s_carry_in = (int)(e0 >> kCarryShift);
-                }
-            }
-            __syncthreads();
-            if (s_abort)
-                return;
-            k_excl = s_kexcl;
-            c_t = s_carry_in;
-            const uint32_t me = tsel<LT>(thread_ex, (uint32_t)c_t);
+            const uint32_t me = tsel<LT>(tcompose<LT>(wpre, ex), (uint32_t)c_t);
             c_i = (int)(me >> 24);
             k_loc = (int)(me & 0xffffffu);
             c_next = last_tile ? 0 : (int)(tsel<LT>(agg, (uint32_t)c_t) >> 24);
             int lst;
             kept = greedy32(m, c_i, L, &lst);
             const int lastq = kept ? seg + lst : kNegInf;
-            // tile totals: kept count and last kept
             int ks = __builtin_popcount(kept), lq = lastq;
 #pragma unroll
             for (int dd = kWave / 2; dd > 0; dd >>= 1) {
@@ -900,53 +843,57 @@ tile_kernel(ReplaceParams p)
                 tile_k += s_wsum[w];
                 tile_last = s_wmax[w] > tile_last ? s_wmax[w] : tile_last;
             }
-            out_len = (int64_t)valid + c_next - c_t + (int64_t)tile_k * diff;
-            staged = p.write_output && r <= kRepLds && out_len + 32 <= kStage;
-            if (staged)
-                stage_output(kept);
-            __syncthreads();
         }
 
-        // ---- records for prefix queries and launch stats -----------------------
-        if (tid == 0) {
-            if constexpr (MODE != kModeBitmap) {
-                TileInfo ti;
-                ti.k_excl = k_excl;
-                ti.last_abs = tile_k > 0 ? start + tile_last : -1;
-                ti.carry_in = c_t;
-                ti.mode = MODE;
-                p.info[tile] = ti;
-            }
-            if (last_tile)
-                p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | (uint64_t)(k_excl + tile_k);
+        // ---- per-tile record (prefix queries) ------------------------------------
+        if (tid == 0 && MODE != kModeBitmap) {
+            TileInfo ti_rec;
+            ti_rec.k_excl = k_run;
+            ti_rec.last_abs = tile_k > 0 ? start + tile_last : -1;
+            ti_rec.carry_in = c_t;
+            ti_rec.mode = MODE;
+            p.info[ti] = ti_rec;
         }
 
-        // ---- output -------------------------------------------------------------
-        uint8_t *const out_first = p.dst + start + c_t + k_excl * diff;   // tile's first output byte
-        if (staged) {
-            // s_in is no longer read: the next tile streams in while wave 0 copies
-            // the staged output out with unaligned 16-byte stores
-            if (need_src && tile + G < nt)
-                prefetch_lds(p, tile + G, tid, halos, s_in);
-            if (wave == 0) {
+        // ---- output ----------------------------------------------------------------
+        if (p.write_output) {
+            const int64_t out_len = (int64_t)valid + c_next - c_t + (int64_t)tile_k * diff;
+            uint8_t *const out_first = p.dst + start + c_t + k_run * diff;   // tile's first output byte
+            if (r <= kRepLds && out_len + 32 <= kStage) {
+                // assemble the tile's output in LDS, then write it with 16-byte stores
+                int o = (seg + c_i - c_t) + k_loc * (int)diff;
+                int j = c_i;
+                uint32_t M = j >= 32 ? 0u : (kept & (0xffffffffu << j));
+                while (j < jend) {
+                    const int q = M ? __builtin_ctz(M) : jend;
+                    if (q > j) {
+                        lds_copy(s_out, o, s_in, in_off + seg + j, q - j);
+                        o += q - j;
+                    }
+                    if (q >= jend)
+                        break;
+                    lds_copy(s_out, o, s_rep, 0, r);
+                    o += r;
+                    j = q + L;
+                    M = j >= 32 ? 0u : (M & (0xffffffffu << j));
+                }
+                __syncthreads();
                 const int len = (int)out_len;
-                for (int c = lane * 16; c < len; c += kWave * 16) {
+                for (int c = tid * 16; c < len; c += kThreads * 16) {
                     const uint4 v = *reinterpret_cast<const uint4 *>(s_out + c);
                     if (c + 16 <= len)
                         __builtin_memcpy(out_first + c, &v, 16);
                     else
                         store_tail(out_first + c, v, len - c);
                 }
-            }
-        } else {
-            if (p.write_output) {
+            } else {
                 uint8_t *out = out_first + (int64_t)(seg + c_i - c_t) + (int64_t)k_loc * diff;
                 int j = c_i;
                 uint32_t M = j >= 32 ? 0u : (kept & (0xffffffffu << j));
                 while (j < jend) {
                     const int q = M ? __builtin_ctz(M) : jend;
                     if (q > j) {
-                        store_from_lds(out, s_in, kHalo + seg + j, q - j);
+                        store_from_lds(out, s_in, in_off + seg + j, q - j);
                         out += q - j;
                     }
                     if (q >= jend)
@@ -960,10 +907,10 @@ tile_kernel(ReplaceParams p)
                     M = j >= 32 ? 0u : (M & (0xffffffffu << j));
                 }
             }
-            __syncthreads();   // s_in fully read
-            if (need_src && tile + G < nt)
-                prefetch_lds(p, tile + G, tid, halos, s_in);
         }
+        k_run += tile_k;
+        c_run = c_next;
+        __syncthreads();   // s_out, s_wsum and s_wtab are reused by the next tile
     }
 }
 
@@ -1118,13 +1065,21 @@ query_kernel(ReplaceParams p, int mode, int64_t limit, int64_t *out2)
         s_part_count = 0;
     }
     __syncthreads();
-    unsigned long long lb = 0;   // last + 1
-    for (int64_t i = tid; i < t; i += kThreads) {
-        const int64_t la = p.info[i].last_abs;
-        if (la + 1 > (int64_t)lb)
-            lb = (unsigned long long)(la + 1);
+    // last kept start before tile t: walk back kThreads tiles at a time and stop at
+    // the first block of tiles that holds one (records are in position order)
+    for (int64_t b = t - 1; b >= 0; b -= kThreads) {
+        const int64_t i = b - tid;
+        if (i >= 0) {
+            const int64_t la = p.info[i].last_abs;
+            if (la >= 0)
+                atomicMax(&s_last_before, (unsigned long long)(la + 1));
+        }
+        __syncthreads();
+        const bool found = s_last_before != 0;
+        __syncthreads();
+        if (found)
+            break;
     }
-    atomicMax(&s_last_before, lb);
     const int64_t gseg = start + (int64_t)tid * 32;
     uint32_t m = 0;
     if (mode == kModeBitmap) {
@@ -1181,24 +1136,25 @@ query_kernel(ReplaceParams p, int mode, int64_t limit, int64_t *out2)
     }
 }
 
-// Largest kept start over all tiles (from the per-tile records), or -1.
+// Largest kept start over all tiles (from the per-tile records) + 1, max-reduced
+// into *out1 (cleared beforehand); 0 when nothing was kept.
 __global__ void __launch_bounds__(kThreads)
-last_kept_kernel(const TileInfo *info, int64_t num_tiles, int64_t *out1)
+last_kept_kernel(const TileInfo *info, int64_t num_tiles, unsigned long long *out1)
 {
-    __shared__ unsigned long long s_best;
-    if (threadIdx.x == 0)
-        s_best = 0;
-    __syncthreads();
-    unsigned long long best = 0;   // last + 1
-    for (int64_t i = threadIdx.x; i < num_tiles; i += kThreads) {
+    unsigned long long best = 0;
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < num_tiles; i += stride) {
         const int64_t la = info[i].last_abs;
         if ((unsigned long long)(la + 1) > best)
             best = (unsigned long long)(la + 1);
     }
-    atomicMax(&s_best, best);
-    __syncthreads();
-    if (threadIdx.x == 0)
-        out1[0] = (int64_t)s_best - 1;
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const unsigned long long o = __shfl_xor(best, d, kWave);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && best)
+        atomicMax(out1, best);
 }
 
 // ---- presence index: 256-bit byte-value mask per chunk -----------------------
@@ -1228,73 +1184,95 @@ presence_kernel(const uint8_t *src, int64_t n, int64_t chunk, uint64_t *masks)
 }
 
 // ---- launchers --------------------------------------------------------------
-// Persistent launch: one workgroup per resident slot (occupancy x CUs, at most one
-// per tile).  Look-back modes launch cooperatively, which guarantees that all
-// workgroups are resident at once.
 template <int MODE, int LT>
-hipError_t launch_tile(const ReplaceParams &p, hipStream_t s)
+hipError_t count_t(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s)
 {
-    constexpr int kMaxDev = 64;
-    static int per_cu[kMaxDev];
-    static int cus[kMaxDev];
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess)
-        return e;
-    if (dev < 0 || dev >= kMaxDev)
-        return hipErrorInvalidDevice;
-    if (per_cu[dev] <= 0) {
-        int nb = 0, nc = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tile_kernel<MODE, LT>, kThreads, 0);
-        if (e != hipSuccess)
-            return e;
-        e = hipDeviceGetAttribute(&nc, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e != hipSuccess)
-            return e;
-        cus[dev] = nc > 0 ? nc : 1;
-        // one slot per CU of margin: with the full reported occupancy not every
-        // workgroup was resident at once on MI355X (look-backs then stall)
-        per_cu[dev] = nb > 1 ? nb - 1 : 1;
-        if (getenv("CR_DEBUG"))
-            fprintf(stderr, "[cr] tile_kernel<%d,%d>: %d workgroups per CU x %d CUs\n", MODE, LT, per_cu[dev],
-                    cus[dev]);
-    }
-    int64_t grid = (int64_t)per_cu[dev] * cus[dev];
-    if (grid > p.num_tiles)
-        grid = p.num_tiles;
-    ReplaceParams pc = p;
-    if (MODE == kModeBitmap) {
-        hipLaunchKernelGGL((tile_kernel<MODE, LT>), dim3((unsigned)grid), dim3(kThreads), 0, s, pc);
-        return hipGetLastError();
-    }
-    // CR_COOPERATIVE=0: plain launch of the same (resident-sized) grid; rocprofv3
-    // crashes at process exit after cooperative launches and loses its trace
-    static const bool coop = !(getenv("CR_COOPERATIVE") && getenv("CR_COOPERATIVE")[0] == '0');
-    if (!coop) {
-        hipLaunchKernelGGL((tile_kernel<MODE, LT>), dim3((unsigned)grid), dim3(kThreads), 0, s, pc);
-        return hipGetLastError();
-    }
-    void *args[] = {&pc};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&tile_kernel<MODE, LT>), dim3((unsigned)grid),
-                                      dim3(kThreads), args, 0, s);
+    hipLaunchKernelGGL((count_kernel<MODE, LT>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
+    return hipGetLastError();
 }
 
-hipError_t launch_border_free(const ReplaceParams &p, hipStream_t s)
+template <int MODE, int LT>
+hipError_t emit_t(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s)
 {
-    return launch_tile<kModeBorderFree, 1>(p, s);
+    hipLaunchKernelGGL((emit_kernel<MODE, LT>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
+    return hipGetLastError();
 }
 
-hipError_t launch_table(const ReplaceParams &p, int table_size, hipStream_t s)
+template <int MODE, int LT>
+hipError_t spine_t(const ReplaceParams &p, int64_t first_group, int64_t groups, int64_t chunk, int last_chunk,
+                   hipStream_t s)
 {
+    hipLaunchKernelGGL((spine_kernel<MODE, LT>), dim3(1), dim3(kSpineThreads), 0, s, p, first_group, groups, chunk,
+                       last_chunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int64_t first_group, int64_t groups,
+                        hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    if (mode == kModeBorderFree)
+        return count_t<kModeBorderFree, 1>(p, first_group, groups, s);
+    if (mode != kModeTable)
+        return hipErrorInvalidValue;
     switch (table_size) {
     case 2:
-        return launch_tile<kModeTable, 2>(p, s);
+        return count_t<kModeTable, 2>(p, first_group, groups, s);
     case 4:
-        return launch_tile<kModeTable, 4>(p, s);
+        return count_t<kModeTable, 4>(p, first_group, groups, s);
     case 8:
-        return launch_tile<kModeTable, 8>(p, s);
+        return count_t<kModeTable, 8>(p, first_group, groups, s);
     case 16:
-        return launch_tile<kModeTable, 16>(p, s);
+        return count_t<kModeTable, 16>(p, first_group, groups, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_t first_group, int64_t groups,
+                        int64_t chunk, int last_chunk, hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    if (mode == kModeBorderFree)
+        return spine_t<kModeBorderFree, 1>(p, first_group, groups, chunk, last_chunk, s);
+    if (mode != kModeTable)
+        return hipErrorInvalidValue;
+    switch (table_size) {
+    case 2:
+        return spine_t<kModeTable, 2>(p, first_group, groups, chunk, last_chunk, s);
+    case 4:
+        return spine_t<kModeTable, 4>(p, first_group, groups, chunk, last_chunk, s);
+    case 8:
+        return spine_t<kModeTable, 8>(p, first_group, groups, chunk, last_chunk, s);
+    case 16:
+        return spine_t<kModeTable, 16>(p, first_group, groups, chunk, last_chunk, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int64_t first_group, int64_t groups,
+                       hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    if (mode == kModeBorderFree)
+        return emit_t<kModeBorderFree, 1>(p, first_group, groups, s);
+    if (mode == kModeBitmap)
+        return emit_t<kModeBitmap, 1>(p, first_group, groups, s);
+    if (mode != kModeTable)
+        return hipErrorInvalidValue;
+    switch (table_size) {
+    case 2:
+        return emit_t<kModeTable, 2>(p, first_group, groups, s);
+    case 4:
+        return emit_t<kModeTable, 4>(p, first_group, groups, s);
+    case 8:
+        return emit_t<kModeTable, 8>(p, first_group, groups, s);
+    case 16:
+        return emit_t<kModeTable, 16>(p, first_group, groups, s);
     default:
         return hipErrorInvalidValue;
     }
@@ -1315,11 +1293,6 @@ hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_bitmap_scatter(const ReplaceParams &p, hipStream_t s)
-{
-    return launch_tile<kModeBitmap, 1>(p, s);
-}
-
 hipError_t launch_empty_pattern(const ReplaceParams &p, hipStream_t s)
 {
     int64_t blocks = (p.n + 1 + kThreads - 1) / kThreads;
@@ -1334,8 +1307,8 @@ hipError_t launch_fill_u64(uint64_t *ptr, uint64_t value, size_t count, hipStrea
     if (count == 0)
         return hipSuccess;
     size_t blocks = (count + kThreads - 1) / kThreads;
-    if (blocks > 65536)
-        blocks = 65536;
+    if (blocks > 4096)
+        blocks = 4096;
     hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, ptr, value, count);
     return hipGetLastError();
 }
@@ -1346,9 +1319,17 @@ hipError_t launch_query(const ReplaceParams &p, int mode, int64_t limit, int64_t
     return hipGetLastError();
 }
 
+// *out1 = last kept start + 1 over the whole pass (0: none)
 hipError_t launch_last_kept(const ReplaceParams &p, int64_t *out1, hipStream_t s)
 {
-    hipLaunchKernelGGL(last_kept_kernel, dim3(1), dim3(kThreads), 0, s, p.info, p.num_tiles, out1);
+    hipError_t e = launch_fill_u64(reinterpret_cast<uint64_t *>(out1), 0, 1, s);
+    if (e != hipSuccess)
+        return e;
+    int64_t blocks = (p.num_tiles + kThreads - 1) / kThreads;
+    if (blocks > 1024)
+        blocks = 1024;
+    hipLaunchKernelGGL(last_kept_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, p.info, p.num_tiles,
+                       reinterpret_cast<unsigned long long *>(out1));
     return hipGetLastError();
 }
 

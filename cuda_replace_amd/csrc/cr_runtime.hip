@@ -1,10 +1,11 @@
 // cr_runtime.hip -- host runtime and C ABI of the gfx950 byte-pattern replace library.
 //
 // A session keeps the buffer resident in HBM as a ping-pong pair (d_buf holds the
-// current contents, d_tmp receives the next pass) plus the scratch the single-pass
-// kernel needs: look-back state words, per-tile records and a small stats block.  Every apply is: upload pattern + replacement (pinned staging,
-// async), one kernel launch (two for long self-overlapping patterns, three for the
-// bitmap fallback), one 32-byte stats readback, swap.
+// current contents, d_tmp receives the next pass) plus the scratch of the chunked
+// kernels: one summary per 16 KiB group, one state per chunk, per-tile records and a
+// small stats block.  Every apply is: upload pattern + replacement (pinned staging,
+// async), a count and an emit launch per chunk (mark + greedy + emit for the bitmap
+// fallback), one 32-byte stats readback, swap.
 //
 // Mirrors the reference's session model (cuda_replace.cu:139-216 CRContext,
 // 997-1140 ensure_* allocators, 2097-2159 apply_once, 2161-2603 exports) with
@@ -26,7 +27,6 @@ namespace {
 
 constexpr size_t kAllocGranule = 2u << 20;   // device buffers grow in 2 MiB steps
 constexpr int kModeEmpty = 4;                // empty pattern (bytes.replace inserts)
-constexpr int kAborted = -1000;              // internal: a look-back launch gave up
 
 struct Session {
     std::mutex mtx;
@@ -41,8 +41,12 @@ struct Session {
     size_t cap_tmp = 0;
     size_t len = 0;
 
-    uint64_t *d_state = nullptr;
-    size_t state_words = 0;
+    uint32_t *d_gsum = nullptr;      // group summaries (count or carry table per group)
+    size_t gsum_words = 0;
+    ChunkState *d_chunks = nullptr;  // state after each chunk
+    size_t chunk_cap = 0;
+    ChunkState *d_pre = nullptr;     // state before each group
+    size_t pre_cap = 0;
     TileInfo *d_info = nullptr;
     size_t info_tiles = 0;
     uint32_t *d_bitmap = nullptr;
@@ -203,16 +207,12 @@ hipError_t ensure_misc(Session *s)
     return hipStreamSynchronize(s->stream);
 }
 
-// next epoch; clear the state words when the 16-bit epoch wraps
+// next epoch; clear the stats words when the 16-bit epoch wraps
 hipError_t next_epoch(Session *s)
 {
     if (s->epoch >= 0xffffu) {
-        if (s->d_state) {
-            hipError_t e = launch_fill_u64(s->d_state, 0, s->state_words, s->stream);
-            if (e != hipSuccess)
-                return e;
-        }
-        hipError_t e = hipMemsetAsync(s->d_stats, 0, sizeof(LaunchStats), s->stream);
+        hipError_t e = launch_fill_u64(reinterpret_cast<uint64_t *>(s->d_stats), 0,
+                                       sizeof(LaunchStats) / sizeof(uint64_t), s->stream);
         if (e != hipSuccess)
             return e;
         s->epoch = 0;
@@ -221,19 +221,46 @@ hipError_t next_epoch(Session *s)
     return hipSuccess;
 }
 
+// Groups per chunk.  By default one chunk holds the whole pass (count, spine and
+// emit each launch once); CR_CHUNK_GROUPS=g splits the pass into chunks of g groups.
+int64_t chunk_groups()
+{
+    static const int64_t v = [] {
+        const char *e = getenv("CR_CHUNK_GROUPS");
+        long long g = e ? atoll(e) : 0;
+        return (int64_t)(g > 0 ? g : INT64_MAX / 4);
+    }();
+    return v;
+}
+
 hipError_t ensure_scratch(Session *s, int64_t num_tiles, int mode, int table_size)
 {
     hipError_t e;
-    // one word (border-free) or one table of table_size words per tile, plus the
-    // fan-out-64 group entries above them
+    // one word (border-free) or one table of table_size words per group
+    const int64_t groups = (num_tiles + kGroupTiles - 1) / kGroupTiles;
     const size_t per = mode == kModeTable ? (size_t)table_size : 1u;
-    size_t words = per * ((size_t)num_tiles + (size_t)num_tiles / 32 + 8);
-    if (s->state_words < words) {
-        size_t cap_bytes = s->state_words * 8;
-        e = grow((void **)&s->d_state, &cap_bytes, words * 8, true, s->stream);
+    const size_t words = per * (size_t)groups;
+    if (s->gsum_words < words) {
+        size_t cap_bytes = s->gsum_words * 4;
+        e = grow((void **)&s->d_gsum, &cap_bytes, words * 4, false, s->stream);
         if (e != hipSuccess)
             return e;
-        s->state_words = cap_bytes / 8;
+        s->gsum_words = cap_bytes / 4;
+    }
+    const size_t chunks = (size_t)((groups + chunk_groups() - 1) / chunk_groups()) + 1;
+    if (s->chunk_cap < chunks) {
+        size_t cap_bytes = s->chunk_cap * sizeof(ChunkState);
+        e = grow((void **)&s->d_chunks, &cap_bytes, chunks * sizeof(ChunkState), false, s->stream);
+        if (e != hipSuccess)
+            return e;
+        s->chunk_cap = cap_bytes / sizeof(ChunkState);
+    }
+    if (s->pre_cap < (size_t)groups) {
+        size_t cap_bytes = s->pre_cap * sizeof(ChunkState);
+        e = grow((void **)&s->d_pre, &cap_bytes, (size_t)groups * sizeof(ChunkState), false, s->stream);
+        if (e != hipSuccess)
+            return e;
+        s->pre_cap = cap_bytes / sizeof(ChunkState);
     }
     if (s->info_tiles < (size_t)num_tiles + 1) {
         size_t cap_bytes = s->info_tiles * sizeof(TileInfo);
@@ -356,6 +383,9 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
 
     CK(upload_pattern(s, pat, L, rep, r));
     const int64_t num_tiles = (n + kTile - 1) / kTile;
+    const int64_t num_groups = (num_tiles + kGroupTiles - 1) / kGroupTiles;
+    const int64_t cg = chunk_groups();
+    const int64_t num_chunks = (num_groups + cg - 1) / cg;
 
     ReplaceParams p{};
     p.src = d_src;
@@ -375,8 +405,9 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     p.mask4 = mask4;
     p.stats = s->d_stats;
     p.num_tiles = num_tiles;
+    p.num_groups = num_groups;
 
-    // read the launch stats; kAborted when a look-back launch gave up
+    // read the pass's stats block (kept count, and the bitmap greedy's last kept)
     auto read_stats = [&](int64_t *kept, int64_t *last) -> int {
         hipError_t e = hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost,
                                       s->stream);
@@ -386,15 +417,9 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         if (e != hipSuccess)
             return (int)e;
         const LaunchStats *h = (const LaunchStats *)(s->h_misc + 64);
-        if ((uint32_t)(h->pad[1] >> kEpochShift) == s->epoch) {
-            if (getenv("CR_DEBUG"))
-                fprintf(stderr, "[cr] look-back gave up (mode %d, %lld tiles): re-running on the bitmap path\n",
-                        mode, (long long)num_tiles);
-            return kAborted;
-        }
         const uint64_t kt = h->k_total, lk = h->last_kept;
         if ((uint32_t)(kt >> kEpochShift) != s->epoch)
-            return (int)hipErrorUnknown;   // the final tile did not report
+            return (int)hipErrorUnknown;   // the final group did not report
         *kept = (int64_t)(kt & ((1ull << kEpochShift) - 1));
         if (last)
             *last = ((uint32_t)(lk >> kEpochShift) == s->epoch)
@@ -405,60 +430,69 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
 
     const int64_t diff = (int64_t)r - L;
     int64_t kept = 0, last = -1;
+    CK(ensure_scratch(s, num_tiles, mode, table_size));
+    p.gsum = s->d_gsum;
+    p.chunks = s->d_chunks;
+    p.chunks_pre = s->d_pre;
+    p.info = s->d_info;
+    p.bitmap = s->d_bitmap;
 
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        if (attempt == 1)
-            mode = kModeBitmap;   // look-back gave up: the bitmap path needs no tile order
-        CK(ensure_scratch(s, num_tiles, mode, table_size));
-        p.state = s->d_state;
-        p.info = s->d_info;
-        p.bitmap = s->d_bitmap;
-
-        auto launch_main = [&](int write) -> hipError_t {
-            hipError_t e = next_epoch(s);
-            if (e != hipSuccess)
-                return e;
-            p.epoch = s->epoch;
-            p.write_output = write;
-            if (mode == kModeBorderFree)
-                return launch_border_free(p, s->stream);
-            return launch_table(p, table_size, s->stream);
-        };
-
-        if (mode == kModeBitmap) {
-            // candidates -> serial greedy (gives the exact count) -> scatter
-            CK(next_epoch(s));
-            p.epoch = s->epoch;
-            if (kernel_ms)
-                CK(hipEventRecord(s->ev0, s->stream));
-            CK(launch_bitmap_mark(p, s->stream));
-            CK(launch_bitmap_greedy(p, s->stream));
-            int rc = read_stats(&kept, &last);
-            if (rc != 0)
-                return rc;
-            const int64_t exact = n + kept * diff;
-            uint8_t *dst = nullptr;
-            rc = dst_for(exact, &dst);
-            if (rc != 0) {
-                res->out_len = exact;
-                return rc;
-            }
-            p.dst = dst;
-            p.out_len = exact;
-            p.write_output = 1;
-            CK(launch_bitmap_scatter(p, s->stream));
-            if (kernel_ms)
-                CK(hipEventRecord(s->ev1, s->stream));
-            CK(hipStreamSynchronize(s->stream));
-            if (kernel_ms)
-                CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
-            break;
+    if (mode == kModeBitmap) {
+        // candidates -> serial greedy (gives the exact count) -> emit
+        CK(next_epoch(s));
+        p.epoch = s->epoch;
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev0, s->stream));
+        CK(launch_bitmap_mark(p, s->stream));
+        CK(launch_bitmap_greedy(p, s->stream));
+        int rc = read_stats(&kept, &last);
+        if (rc != 0)
+            return rc;
+        const int64_t exact = n + kept * diff;
+        uint8_t *dst = nullptr;
+        rc = dst_for(exact, &dst);
+        if (rc != 0) {
+            res->out_len = exact;
+            return rc;
         }
+        p.dst = dst;
+        p.out_len = exact;
+        p.write_output = 1;
+        CK(launch_emit(p, kModeBitmap, 0, 0, num_groups, s->stream));
+        if (kernel_ms)
+            CK(hipEventRecord(s->ev1, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        if (kernel_ms)
+            CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
+    } else {
+        // every chunk: count, spine, emit (stream order hands each chunk's final
+        // state to the next chunk's spine)
+        auto run_chunks = [&](bool emit) -> hipError_t {
+            for (int64_t c = 0; c < num_chunks; ++c) {
+                const int64_t g0 = c * cg;
+                const int64_t gn = num_groups - g0 < cg ? num_groups - g0 : cg;
+                hipError_t e = launch_count(p, mode, table_size, g0, gn, s->stream);
+                if (e != hipSuccess)
+                    return e;
+                e = launch_spine(p, mode, table_size, g0, gn, c, c == num_chunks - 1, s->stream);
+                if (e != hipSuccess)
+                    return e;
+                if (emit) {
+                    e = launch_emit(p, mode, table_size, g0, gn, s->stream);
+                    if (e != hipSuccess)
+                        return e;
+                }
+            }
+            return hipSuccess;
+        };
+        CK(next_epoch(s));
+        p.epoch = s->epoch;
+        p.write_output = 1;
 
         // output size: shrink never grows; expand uses the non-overlapping worst
-        // case when affordable, otherwise a count-only pass first
+        // case when affordable, otherwise the exact size from count + spine first
         int64_t need = n;
-        int64_t exact = -1;
+        bool counted = false;
         if (diff > 0) {
             const int64_t span = floor_pos < n ? n - floor_pos : 0;
             const int64_t worst = n + (span / L) * diff;
@@ -468,43 +502,44 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             if (worst <= afford) {
                 need = worst;
             } else {
-                CK(launch_main(0));
+                if (kernel_ms)
+                    CK(hipEventRecord(s->ev0, s->stream));
+                CK(run_chunks(false));
                 int rc = read_stats(&kept, nullptr);
-                if (rc == kAborted)
-                    continue;
                 if (rc != 0)
                     return rc;
                 need = n + kept * diff;
-                exact = need;
+                counted = true;
             }
         }
         uint8_t *dst = nullptr;
         int rc = dst_for(need, &dst);
         if (rc != 0) {
-            res->out_len = exact >= 0 ? exact : need;
+            res->out_len = need;
             return rc;
         }
         p.dst = dst;
-        if (kernel_ms)
-            CK(hipEventRecord(s->ev0, s->stream));
-        CK(launch_main(1));
+        if (counted) {
+            CK(launch_emit(p, mode, table_size, 0, num_groups, s->stream));
+        } else {
+            if (kernel_ms)
+                CK(hipEventRecord(s->ev0, s->stream));
+            CK(run_chunks(true));
+        }
         if (kernel_ms)
             CK(hipEventRecord(s->ev1, s->stream));
         if (want_last)
             CK(launch_last_kept(p, s->d_query, s->stream));
         rc = read_stats(&kept, nullptr);
-        if (rc == kAborted)
-            continue;
         if (rc != 0)
             return rc;
         if (want_last) {
             CK(hipMemcpyAsync(s->h_misc + 128, s->d_query, 8, hipMemcpyDeviceToHost, s->stream));
             CK(hipStreamSynchronize(s->stream));
-            last = ((const int64_t *)(s->h_misc + 128))[0];
+            last = ((const int64_t *)(s->h_misc + 128))[0] - 1;
         }
         if (kernel_ms)
             CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
-        break;
     }
     if (mode_out)
         *mode_out = mode;
@@ -599,7 +634,7 @@ void destroy(Session *s)
     DeviceGuard g(s->device);
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
-    void *dev[] = {s->d_buf, s->d_tmp, s->d_state, s->d_info, s->d_bitmap, s->d_patrep, s->d_presence, s->d_misc};
+    void *dev[] = {s->d_buf, s->d_tmp, s->d_gsum, s->d_chunks, s->d_pre, s->d_info, s->d_bitmap, s->d_patrep, s->d_presence, s->d_misc};
     for (void *p : dev)
         if (p)
             (void)hipFree(p);
