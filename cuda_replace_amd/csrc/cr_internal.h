@@ -21,7 +21,7 @@ constexpr int kTile = kThreads * kBytesPerThread;   // 8192 source bytes per til
 constexpr int kHalo = 256;                    // LDS halo on each side of a group
 constexpr int kMaxSinglePassPat = kHalo;      // chunked kernels take L <= 256
 constexpr int kMaxTablePat = 16;              // carry tables (self-overlapping patterns) take L <= 16
-constexpr int kStage = kTile + kTile / 8 + 64; // LDS stage for a tile's output (bytes)
+constexpr int kStage = kTile + kTile / 8 + 64; // LDS image of a tile's output (bytes)
 
 // A group is kGroupTiles consecutive tiles handled by one workgroup; a chunk is
 // a run of consecutive groups handled by one count, one spine and one emit launch.

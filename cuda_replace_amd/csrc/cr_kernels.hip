@@ -139,52 +139,6 @@ __device__ __forceinline__ void store_tail(uint8_t *dst, uint4 v, int len)
         *dst = (uint8_t)lo;
 }
 
-// Copies len bytes inside LDS without loop-carried LDS round trips: up to 3
-// predicated head bytes until the destination is 4-byte aligned, then blocks of up
-// to 8 aligned dword writes whose source dwords (re-aligned with alignbyte) are
-// all read first, then up to 3 predicated tail bytes.  Reads at most 4 bytes past
-// the source range.
-__device__ __forceinline__ void lds_copy(uint8_t *D, int dst, const uint8_t *S, int src, int len)
-{
-    const int h0 = (-dst) & 3;
-    const int h = h0 < len ? h0 : len;
-    if (h > 0)
-        D[dst] = S[src];
-    if (h > 1)
-        D[dst + 1] = S[src + 1];
-    if (h > 2)
-        D[dst + 2] = S[src + 2];
-    dst += h;
-    src += h;
-    len -= h;
-    const uint32_t sh = (uint32_t)(src & 3);
-    const uint32_t *sp = reinterpret_cast<const uint32_t *>(S + (src & ~3));
-    uint32_t *dp = reinterpret_cast<uint32_t *>(D + dst);
-    int nd = len >> 2;
-    while (nd > 0) {
-        uint32_t v[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-            v[k] = k <= nd ? sp[k] : 0u;
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (k < nd)
-                dp[k] = __builtin_amdgcn_alignbyte(v[k + 1], v[k], sh);
-        const int blk = nd < 8 ? nd : 8;
-        sp += blk;
-        dp += blk;
-        nd -= blk;
-    }
-    const int t = len & 3;
-    const int ts = src + (len & ~3), td = dst + (len & ~3);
-    if (t > 0)
-        D[td] = S[ts];
-    if (t > 1)
-        D[td + 1] = S[ts + 1];
-    if (t > 2)
-        D[td + 2] = S[ts + 2];
-}
-
 // Copies len bytes from LDS (byte offset src) to HBM at dst: unaligned 16-byte
 // stores (full speed on gfx950), then an exact tail.  Reads up to 19 bytes past the
 // source range.
@@ -196,6 +150,26 @@ __device__ __forceinline__ void store_from_lds(uint8_t *dst, const uint8_t *lds,
     }
     if (len > 0)
         store_tail(dst, lds_read16(lds, src), len);
+}
+
+// Unaligned 16-byte LDS accesses: gfx950 runs with unaligned LDS access enabled, so
+// these compile to single ds_read_b128 / ds_write_b128 at any byte address.
+__device__ __forceinline__ uint4 lds_load16(const uint8_t *p)
+{
+    uint4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+__device__ __forceinline__ void lds_store16(uint8_t *p, uint4 v) { __builtin_memcpy(p, &v, 16); }
+
+// Copies len bytes between LDS buffers (any alignment): 16-byte moves, then an exact
+// tail.  Reads up to 15 bytes past the source range.
+__device__ __forceinline__ void lds_move(uint8_t *dst, const uint8_t *src, int len)
+{
+    for (; len >= 16; len -= 16, src += 16, dst += 16)
+        lds_store16(dst, lds_load16(src));
+    if (len > 0)
+        store_tail(dst, lds_load16(src), len);
 }
 
 // Copies len bytes HBM -> HBM with unaligned 16-byte accesses (long replacements).
@@ -274,6 +248,16 @@ __device__ __forceinline__ Table64<LT> wave_compose(Table64<LT> t, int lane)
             t = tcompose64<LT>(t, o);
     }
     return t;
+}
+
+template <int LT>
+__device__ __forceinline__ Table64<LT> wave_broadcast_last(const Table64<LT> &t)
+{
+    Table64<LT> r;
+#pragma unroll
+    for (int c = 0; c < LT; ++c)
+        r.e[c] = __shfl(t.e[c], kWave - 1, kWave);
+    return r;
 }
 
 template <int LT>
@@ -357,11 +341,13 @@ __device__ __forceinline__ uint32_t cand_mask(const ReplaceParams &p, const uint
     for (int k = 0; k < 9; ++k)
         d[k] = w32[k];
     const uint32_t pat4 = p.pat4, mask4 = p.mask4;
+    // bit j <- window at j equals the pattern head; m + m + cmp is one v_addc per
+    // position (the compare's lane mask is the carry in)
     uint32_t m = 0;
 #pragma unroll
     for (int j = 31; j >= 0; --j) {
-        const uint32_t w = __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
-        m = (m << 1) | (uint32_t)(((w ^ pat4) & mask4) == 0u);
+        const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
+        m = m + m + (uint32_t)(((w ^ pat4) & mask4) == 0u);
     }
     const int64_t lo = p.floor - gseg;
     const int64_t hi = p.n - (int64_t)L - gseg;
@@ -482,23 +468,29 @@ count_kernel(ReplaceParams p, int64_t first_group)
 
 // ---- spine: exclusive kept state of every group of a chunk -------------------
 // One workgroup of kSpineThreads walks the chunk's group summaries in passes of
-// kSpineThreads * V groups (V = kSpineWords / words per summary, so a pass stages
-// 64 KiB): a coalesced load into LDS, each thread folds its V consecutive groups,
-// a block-wide ordered scan of the thread folds, then each thread writes the
-// exclusive state (kept before the group, carry into it) of its groups.  The state after the chunk goes to chunks[chunk + 1]; the last
+// kSpineThreads * V groups.  Wave w owns the pass's groups [w*64V, (w+1)*64V) and
+// lane l their groups w*64V + 64j + l (j < V), so every load and store of the wave
+// is one coalesced run.  Per j the wave scans its 64 summaries in order (shuffles),
+// carrying the running state to j+1; the wave totals are then composed across
+// waves through LDS and each group's exclusive state (kept before it, carry into
+// it) is written.  The state after the chunk goes to chunks[chunk + 1]; the last
 // chunk also publishes the pass's kept total.
 constexpr int kSpineThreads = 1024;
 constexpr int kSpineWaves = kSpineThreads / kWave;
-constexpr int kSpineWords = 16;
+
+template <int W>
+struct SpineShape {
+    static constexpr int V = W <= 2 ? 16 : (W <= 8 ? 8 : 4);   // groups per lane and pass
+};
 
 template <int MODE, int LT>
 __global__ void __launch_bounds__(kSpineThreads)
 spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk, int last_chunk)
 {
     constexpr int W = MODE == kModeTable ? LT : 1;   // words per group summary
-    constexpr int V = kSpineWords / W;                // groups per thread and pass
-    constexpr int kPass = kSpineThreads * V;
-    __shared__ uint32_t s_v[kPass * W + kSpineThreads];   // padded: no bank conflicts on the fold
+    constexpr int V = SpineShape<W>::V;
+    constexpr int64_t kBlock = (int64_t)kWave * V;   // groups per wave and pass
+    constexpr int64_t kPass = kBlock * kSpineWaves;
     __shared__ uint64_t s_wt[kSpineWaves][LT];
     __shared__ int64_t s_wsum[kSpineWaves];
 
@@ -511,32 +503,32 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
         k = p.chunks[chunk].k_base;
         carry = p.chunks[chunk].carry;
     }
-    // LDS slot of word w of value i: one pad word per thread's V values, so the
-    // threads' folds read with an odd stride (no bank conflicts)
-    auto slot = [](int i, int w) { return (i * W + w) + (i / V); };
 
     for (int64_t base = 0; base < groups; base += kPass) {
-        const int cnt = (int)(groups - base < kPass ? groups - base : kPass);
-        for (int i = tid; i < cnt * W; i += kSpineThreads)
-            s_v[slot(i / W, i % W)] = p.gsum[(first_group + base) * W + i];
-        __syncthreads();
-        const int v0 = tid * V;
+        const int64_t wb = base + (int64_t)wave * kBlock;   // wave's first group (chunk-relative)
         if constexpr (MODE != kModeTable) {
-            int64_t mine = 0;
+            // kept counts: per j an inclusive wave scan, offset by the running sum
+            uint32_t v[V], ex[V];
+            uint32_t run = 0;
 #pragma unroll
-            for (int j = 0; j < V; ++j)
-                if (v0 + j < cnt)
-                    mine += s_v[slot(v0 + j, 0)];
-            // block exclusive scan of the thread sums
-            int64_t inc = mine;
-#pragma unroll
-            for (int d = 1; d < kWave; d <<= 1) {
-                const int64_t u = __shfl_up(inc, d, kWave);
-                if (lane >= d)
-                    inc += u;
+            for (int j = 0; j < V; ++j) {
+                const int64_t gi = wb + (int64_t)j * kWave + lane;
+                v[j] = gi < groups ? p.gsum[first_group + gi] : 0u;
             }
-            if (lane == kWave - 1)
-                s_wsum[wave] = inc;
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                uint32_t inc = v[j];
+#pragma unroll
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const uint32_t u = __shfl_up(inc, d, kWave);
+                    if (lane >= d)
+                        inc += u;
+                }
+                ex[j] = run + inc - v[j];
+                run += __shfl(inc, kWave - 1, kWave);
+            }
+            if (lane == 0)
+                s_wsum[wave] = run;
             __syncthreads();
             int64_t before = 0, total = 0;
             for (int w = 0; w < kSpineWaves; ++w) {
@@ -544,50 +536,53 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
                     before += s_wsum[w];
                 total += s_wsum[w];
             }
-            int64_t run = k + before + inc - mine;
+#pragma unroll
             for (int j = 0; j < V; ++j) {
-                if (v0 + j < cnt) {
+                const int64_t gi = wb + (int64_t)j * kWave + lane;
+                if (gi < groups) {
                     ChunkState cs;
-                    cs.k_base = run;
+                    cs.k_base = k + before + ex[j];
                     cs.carry = 0;
                     cs.pad = 0;
-                    p.chunks_pre[first_group + base + v0 + j] = cs;
-                    run += s_v[slot(v0 + j, 0)];
+                    p.chunks_pre[first_group + gi] = cs;
                 }
             }
             k += total;
         } else {
-            // fold my groups' tables, then an ordered block scan of the folds
-            Table64<LT> mine = tidentity64<LT>();
-            for (int j = 0; j < V; ++j) {
-                if (v0 + j < cnt) {
-                    Table<LT> t;
+            // carry tables, two sweeps over j: first the wave's total table, then,
+            // once the state entering the wave's block is known, each group's
+            // exclusive state from an ordered wave scan (summaries reloaded from L2)
+            auto load_table = [&](int64_t gi) {
+                Table<LT> t = tidentity<LT>();
+                if (gi < groups) {
 #pragma unroll
                     for (int c = 0; c < LT; ++c)
-                        t.e[c] = s_v[slot(v0 + j, c)];
-                    mine = tcompose64<LT>(mine, widen<LT>(t));
+                        t.e[c] = p.gsum[(first_group + gi) * LT + c];
                 }
-            }
-            Table64<LT> inc = mine;
+                return widen<LT>(t);
+            };
+            auto wave_scan = [&](Table64<LT> inc) {
 #pragma unroll
-            for (int d = 1; d < kWave; d <<= 1) {
-                Table64<LT> nb;
+                for (int d = 1; d < kWave; d <<= 1) {
+                    Table64<LT> nb;
+#pragma unroll
+                    for (int c = 0; c < LT; ++c)
+                        nb.e[c] = __shfl_up(inc.e[c], d, kWave);
+                    if (lane >= d)
+                        inc = tcompose64<LT>(nb, inc);
+                }
+                return inc;
+            };
+            Table64<LT> run = tidentity64<LT>();
+#pragma unroll 1
+            for (int j = 0; j < V; ++j) {
+                const Table64<LT> t = load_table(wb + (int64_t)j * kWave + lane);
+                run = tcompose64<LT>(run, wave_broadcast0<LT>(wave_compose<LT>(t, lane)));
+            }
+            if (lane == 0) {
 #pragma unroll
                 for (int c = 0; c < LT; ++c)
-                    nb.e[c] = __shfl_up(inc.e[c], d, kWave);
-                if (lane >= d)
-                    inc = tcompose64<LT>(nb, inc);
-            }
-            Table64<LT> ex;
-#pragma unroll
-            for (int c = 0; c < LT; ++c) {
-                const uint64_t v = __shfl_up(inc.e[c], 1, kWave);
-                ex.e[c] = lane == 0 ? ((uint64_t)c << kCarryShift) : v;
-            }
-            if (lane == kWave - 1) {
-#pragma unroll
-                for (int c = 0; c < LT; ++c)
-                    s_wt[wave][c] = inc.e[c];
+                    s_wt[wave][c] = run.e[c];
             }
             __syncthreads();
             Table64<LT> wpre = tidentity64<LT>(), tot = tidentity64<LT>();
@@ -600,30 +595,36 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
                     wpre = tcompose64<LT>(wpre, wt);
                 tot = tcompose64<LT>(tot, wt);
             }
-            const uint64_t e = tsel64<LT>(tcompose64<LT>(wpre, ex), (uint32_t)carry);
-            int64_t rk = k + (int64_t)(e & kCountMask);
-            int rc = (int)(e >> kCarryShift);
+            const uint64_t ew = tsel64<LT>(wpre, (uint32_t)carry);
+            int64_t kw = k + (int64_t)(ew & kCountMask);   // state entering the j-block
+            uint32_t cw = (uint32_t)(ew >> kCarryShift);
+#pragma unroll 1
             for (int j = 0; j < V; ++j) {
-                if (v0 + j < cnt) {
-                    ChunkState cs;
-                    cs.k_base = rk;
-                    cs.carry = rc;
-                    cs.pad = 0;
-                    p.chunks_pre[first_group + base + v0 + j] = cs;
-                    Table<LT> t;
+                const int64_t gi = wb + (int64_t)j * kWave + lane;
+                const Table64<LT> inc = wave_scan(load_table(gi));
+                uint64_t e = (uint64_t)cw << kCarryShift;   // lane 0: nothing before it
 #pragma unroll
-                    for (int c = 0; c < LT; ++c)
-                        t.e[c] = s_v[slot(v0 + j, c)];
-                    const uint32_t te = tsel<LT>(t, (uint32_t)rc);
-                    rk += (int64_t)(te & 0xffffffu);
-                    rc = (int)(te >> 24);
+                for (int c = 0; c < LT; ++c) {
+                    const uint64_t u = __shfl_up(inc.e[c], 1, kWave);
+                    if (lane > 0 && (uint32_t)c == cw)
+                        e = u;
                 }
+                if (gi < groups) {
+                    ChunkState cs;
+                    cs.k_base = kw + (int64_t)(e & kCountMask);
+                    cs.carry = (int)(e >> kCarryShift);
+                    cs.pad = 0;
+                    p.chunks_pre[first_group + gi] = cs;
+                }
+                const uint64_t el = tsel64<LT>(wave_broadcast_last<LT>(inc), cw);
+                kw += (int64_t)(el & kCountMask);
+                cw = (uint32_t)(el >> kCarryShift);
             }
             const uint64_t te = tsel64<LT>(tot, (uint32_t)carry);
             k += (int64_t)(te & kCountMask);
             carry = (int)(te >> kCarryShift);
         }
-        __syncthreads();   // s_v and the wave slots are reused by the next pass
+        __syncthreads();   // the wave slots are reused by the next pass
     }
     if (tid == 0) {
         ChunkState cs;
@@ -688,6 +689,7 @@ emit_kernel(ReplaceParams p, int64_t first_group)
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    const uint4 rep_v = lds_load16(s_rep);   // the replacement's first 16 bytes
 
     const int seg = tid * kBytesPerThread;
     for (int i = 0; i < kGroupTiles; ++i) {
@@ -855,24 +857,33 @@ emit_kernel(ReplaceParams p, int64_t first_group)
             p.info[ti] = ti_rec;
         }
 
-        // ---- output ----------------------------------------------------------------
+        // ---- output ------------------------------------------------------------------
+        // Each thread places its source runs and replacements into the tile's output
+        // image in LDS (unaligned 16-byte LDS moves, exact tails), then the workgroup
+        // writes the image out with contiguous 16-byte stores.  A tile whose output
+        // does not fit the stage (or a replacement longer than kRepLds) is written
+        // straight to HBM by each thread instead.
         if (p.write_output) {
             const int64_t out_len = (int64_t)valid + c_next - c_t + (int64_t)tile_k * diff;
             uint8_t *const out_first = p.dst + start + c_t + k_run * diff;   // tile's first output byte
             if (r <= kRepLds && out_len + 32 <= kStage) {
-                // assemble the tile's output in LDS, then write it with 16-byte stores
                 int o = (seg + c_i - c_t) + k_loc * (int)diff;
                 int j = c_i;
                 uint32_t M = j >= 32 ? 0u : (kept & (0xffffffffu << j));
                 while (j < jend) {
                     const int q = M ? __builtin_ctz(M) : jend;
                     if (q > j) {
-                        lds_copy(s_out, o, s_in, in_off + seg + j, q - j);
+                        lds_move(s_out + o, s_in + in_off + seg + j, q - j);
                         o += q - j;
                     }
                     if (q >= jend)
                         break;
-                    lds_copy(s_out, o, s_rep, 0, r);
+                    if (r < 16)
+                        store_tail(s_out + o, rep_v, r);
+                    else if (r == 16)
+                        lds_store16(s_out + o, rep_v);
+                    else
+                        lds_move(s_out + o, s_rep, r);
                     o += r;
                     j = q + L;
                     M = j >= 32 ? 0u : (M & (0xffffffffu << j));
