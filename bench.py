@@ -5,14 +5,14 @@ Workload (one "step" = three replace passes over HBM-resident inputs):
   * the same text, fragment -> 1 byte (shrink)
   * 1 GiB of b"a", b"aa" -> b"X" (every position a candidate, maximal overlap)
 Each pass reads a resident source buffer and writes a resident destination buffer
-through cr_replace_device (the session runtime's single-pass kernel).  Every
-output is compared bit-exactly with CPython bytes.replace in the same run.
+through cr_replace_device (count, spine and emit launches on the session stream).
+Every output is compared bit-exactly with CPython bytes.replace in the same run.
 
 value    = input bytes of all passes / wall time of the K timed steps (GB/s, 1e9)
-roofline = the dominant kernel's algorithmic bytes (n_in + n_out per launch) over
-           its average launch time from HIP events on the library's stream,
-           against 8 TB/s; traffic comes from the committed rocprofv3 counter
-           summary (profiles/traffic.json) when present.
+roofline = algorithmic bytes of a border-free pass (n_in + n_out) over the pass's
+           device time from HIP events around its launches on the library's
+           stream, against 8 TB/s; traffic (HBM bytes per pass, counter-calibrated)
+           comes from the committed rocprofv3 summary (profiles/traffic.json).
 cpu_baseline = bytes.replace, single thread, on the same three inputs (one pass
            each, best of two), plus a fork-based per-core chunked variant.
 
@@ -223,7 +223,7 @@ def main() -> int:
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("tile_kernel_border_free_bytes_per_launch")
+            traffic = json.load(open(tpath)).get("border_free_pass_bytes")
         except (OSError, ValueError):
             traffic = None
     cpu_in = 3 * n
@@ -247,7 +247,8 @@ def main() -> int:
         "bit_exact": exact,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": traffic,
-                     "kernel": "tile_kernel<border-free> (text expand + shrink launches)"},
+                     "kernel": "one border-free replace pass = count_kernel + spine_kernel + emit_kernel "
+                               "(text expand and shrink), timed by HIP events around the three launches"},
         "cpu_baseline": {"value": round(cpu_in / cpu_t / 1e9, 3), "unit": "GB/s", "cores": 1,
                          "kind": "port",
                          "sample": f"CPython bytes.replace, single thread, one pass over each of the "
