@@ -26,8 +26,9 @@ Parity status of each function (pinned by tests/test_oracle.py):
 * ``kept_positions``     -- pinned: CPython ``bytes.find`` loop, ``bytes.count``
   for border-free patterns, closed forms on periodic inputs, brute force.
 * ``apply_seeded``       -- pinned: ``S[:F] + S[F:].replace(P, R)`` with
-  ``F = max(0, seed + L)`` by brute force over every input <= 7 bytes on
-  ``{a, b, NUL}`` with every seed in ``[-L-2, n+1]``.
+  ``F = max(0, seed + L)`` by brute force over every input of up to 5 bytes on
+  ``{a, b, NUL}`` (plus 600 seeded random inputs of 6-16 bytes) with every seed
+  in ``[-L-2, n+1]``.
 * ``query_prefix``       -- pinned through ``kept_positions``.
 * ``apply_batch``        -- ``functools.reduce`` of ``bytes.replace``.
 * ``streaming``          -- pinned: equals ``bytes.replace`` of the whole input
