@@ -4,8 +4,8 @@ The compute path is libcuda_replace.so (hand-written HIP kernels + C++ runtime,
 built in-tree by ``python -m cuda_replace_amd.build``).  This package exposes it
 through the reference wrapper's API.
 """
-from .wrapper import (DEFAULT_LIB, REQUIRED_SYMBOLS, CudaReplaceLib, Session,  # noqa: F401
-                      gpu_replace_streaming, replace_unified)
+from .wrapper import (DEFAULT_LIB, REQUIRED_SYMBOLS, BatchEngine, CudaReplaceLib,  # noqa: F401
+                      Session, gpu_replace_streaming, replace_unified)
 
-__all__ = ["CudaReplaceLib", "Session", "replace_unified", "gpu_replace_streaming",
+__all__ = ["CudaReplaceLib", "Session", "BatchEngine", "replace_unified", "gpu_replace_streaming",
            "DEFAULT_LIB", "REQUIRED_SYMBOLS"]

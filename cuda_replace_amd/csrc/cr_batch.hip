@@ -1,0 +1,667 @@
+// cr_batch.hip -- batch engine: many independent documents of uneven size, each with
+// its own ordered list of (pattern, replacement) pairs.
+//
+// Round j applies pair j of every document that has one.  A round is: upload the
+// round's document descriptors and pattern pool; per pattern class (border-free, or
+// carry tables of 2/4/8/16 states) one batched count launch over all groups of the
+// class's documents and one spine launch (one wave per document); read back the
+// documents' kept counts; pack the new lengths into the destination arena on the
+// host; per class one batched emit launch.  Documents whose pattern is empty or needs
+// the bitmap path run one at a time after the batched emit.  Documents that finish
+// in a round are written into the final arena, packed in finishing order, so no
+// later round overwrites them; one device-to-host copy returns them all.
+//
+// The engine keeps its arenas, descriptor buffers and pinned staging resident across
+// runs (sized by the largest run so far).  No counterpart in the reference, whose
+// callers loop over cuda_replace_unified (cuda_replace.cu:2579-2601) per document
+// and pair.
+#include "cr_internal.h"
+#include "cr_pattern.h"
+#include "../../include/cuda_replace.h"
+
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <new>
+#include <string.h>
+#include <thread>
+#include <vector>
+
+using namespace cr;
+
+namespace {
+
+constexpr size_t kDocAlign = 256;      // documents start at 256-byte boundaries in the arenas
+constexpr size_t kGranule = 2u << 20;  // device buffers grow in 2 MiB steps
+constexpr int kClasses = 5;            // border-free, tables of 2, 4, 8, 16 states
+constexpr int kOther = kClasses;       // empty pattern or bitmap path: one at a time
+constexpr int kMaxSummaryWords = kMaxTablePat;
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+#define CK(expr)                                  \
+    do {                                          \
+        hipError_t e_ = (expr);                   \
+        if (e_ != hipSuccess)                     \
+            return (int)e_;                       \
+    } while (0)
+
+struct DevBuf {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+};
+struct HostBuf {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+};
+
+// Grows b to at least need bytes; preserve copies the first `used` bytes over.
+hipError_t dev_reserve(DevBuf &b, size_t need, bool preserve, size_t used, hipStream_t s)
+{
+    if (b.p && b.cap >= need)
+        return hipSuccess;
+    size_t nc = align_up(std::max(need, b.cap + b.cap / 2), kGranule);
+    uint8_t *np = nullptr;
+    hipError_t e = hipMalloc((void **)&np, nc);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        nc = align_up(need > 0 ? need : 1, kGranule);
+        e = hipMalloc((void **)&np, nc);
+        if (e != hipSuccess)
+            return e;
+    }
+    if (b.p) {
+        if (preserve && used > 0) {
+            e = hipMemcpyAsync(np, b.p, used, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) {
+                (void)hipFree(np);
+                return e;
+            }
+        }
+        e = hipStreamSynchronize(s);   // queued work may still read the old buffer
+        (void)hipFree(b.p);
+        if (e != hipSuccess) {
+            b.p = np;
+            b.cap = nc;
+            return e;
+        }
+    }
+    b.p = np;
+    b.cap = nc;
+    return hipSuccess;
+}
+
+hipError_t host_reserve(HostBuf &b, size_t need)
+{
+    if (b.p && b.cap >= need)
+        return hipSuccess;
+    const size_t nc = align_up(std::max(need, b.cap + b.cap / 2), 1u << 20);
+    void *np = nullptr;
+    hipError_t e = hipHostMalloc(&np, nc, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return e;
+    if (b.p)
+        (void)hipHostFree(b.p);
+    b.p = (uint8_t *)np;
+    b.cap = nc;
+    return hipSuccess;
+}
+
+struct Piece {
+    size_t dst_off;
+    const uint8_t *src;
+    size_t len;
+};
+
+// Host memcpy of many pieces into one buffer, split over threads by bytes.
+void copy_pieces(uint8_t *dst, const std::vector<Piece> &pieces, size_t total)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthreads = total < (32u << 20) ? 1 : (int)std::min<unsigned>(hw ? hw : 1, 8);
+    auto run = [&](size_t lo, size_t hi) {   // byte range [lo, hi) of the concatenation
+        size_t acc = 0;
+        for (const Piece &pc : pieces) {
+            const size_t a = std::max(lo, acc), b = std::min(hi, acc + pc.len);
+            if (a < b)
+                memcpy(dst + pc.dst_off + (a - acc), pc.src + (a - acc), b - a);
+            acc += pc.len;
+            if (acc >= hi)
+                break;
+        }
+    };
+    if (nthreads <= 1) {
+        run(0, total);
+        return;
+    }
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; ++t)
+        ts.emplace_back(run, total * t / nthreads, total * (t + 1) / nthreads);
+    for (auto &t : ts)
+        t.join();
+}
+
+struct DocState {
+    uint8_t *cur = nullptr;   // device location of the current contents
+    int64_t len = 0;
+    int64_t pair0 = 0;        // first pair of the document in the flattened lists
+    int npairs = 0;
+    size_t fin_off = 0;       // final arena offset once finished
+};
+
+struct Engine {
+    std::mutex mtx;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf arena[2], fin;
+    DevBuf d_docs, d_groups, d_cdocs, d_pool, d_gsum, d_pre, d_kept, d_other;
+    // descriptor staging alternates by round parity: round j+1 fills one copy while
+    // round j's uploads may still read the other (round j+1's sync orders them)
+    HostBuf h_docs[2], h_groups[2], h_cdocs[2], h_pool[2];
+    HostBuf h_in, h_kept, h_out, h_misc;
+};
+
+struct DeviceGuard {
+    int prev = -1;
+    bool changed = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev)
+            changed = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (changed)
+            (void)hipSetDevice(prev);
+    }
+};
+
+int class_of(int mode, int lt)
+{
+    if (mode == kModeBorderFree)
+        return 0;
+    if (mode == kModeTable)
+        return lt == 2 ? 1 : lt == 4 ? 2 : lt == 8 ? 3 : 4;
+    return kOther;
+}
+
+// One document of a round through the single-buffer kernels: empty pattern, or the
+// bitmap path (candidate bitmap, serial greedy, emit).  Returns the output length
+// via *out_len; dst_for(len) supplies the destination.
+template <class DstFn>
+int run_other(Engine *E, const DocState &ds, const uint8_t *pat, int L, const uint8_t *rep, int r,
+              int64_t *out_len, DstFn &&dst_for)
+{
+    hipStream_t s = E->stream;
+    const int64_t n = ds.len;
+    // device copies of pattern and replacement at the front of the scratch
+    const size_t pr = align_up((size_t)L + (size_t)r + 16, 256);
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    const int64_t groups = (tiles + kGroupTiles - 1) / kGroupTiles;
+    const size_t bitmap_bytes = align_up((size_t)(tiles > 0 ? tiles : 1) * (kTile / 8), 256);
+    const size_t info_bytes = align_up((size_t)(tiles + 1) * sizeof(TileInfo), 256);
+    const size_t need = pr + 256 + (L > 0 ? bitmap_bytes + info_bytes : 0);
+    CK(dev_reserve(E->d_other, need, false, 0, s));
+    CK(host_reserve(E->h_misc, pr + 256));
+    if (L > 0)
+        memcpy(E->h_misc.p, pat, (size_t)L);
+    if (r > 0)
+        memcpy(E->h_misc.p + L, rep, (size_t)r);
+    if (L + r > 0)
+        CK(hipMemcpyAsync(E->d_other.p, E->h_misc.p, (size_t)L + r, hipMemcpyHostToDevice, s));
+    ReplaceParams p{};
+    p.src = ds.cur;
+    p.n = n;
+    p.floor = 0;
+    p.out_len = -1;
+    p.pat = E->d_other.p;
+    p.rep = E->d_other.p + L;
+    p.L = L;
+    p.r = r;
+    pattern_head(pat, L, &p.pat4, &p.mask4);
+    p.stats = reinterpret_cast<LaunchStats *>(E->d_other.p + pr);
+    p.num_tiles = tiles;
+    p.num_groups = groups;
+    p.epoch = 1;
+    if (L == 0) {
+        const int64_t len = n + (n + 1) * (int64_t)r;
+        uint8_t *dst = nullptr;
+        int rc = dst_for(len, &dst);
+        if (rc != 0)
+            return rc;
+        p.dst = dst;
+        if (r == 0) {
+            if (n > 0)
+                CK(hipMemcpyAsync(dst, ds.cur, (size_t)n, hipMemcpyDeviceToDevice, s));
+        } else {
+            CK(launch_empty_pattern(p, s));
+        }
+        *out_len = len;
+        return (int)hipStreamSynchronize(s);   // the pinned staging is reused next call
+    }
+    if (n == 0) {
+        uint8_t *dst = nullptr;
+        *out_len = 0;
+        return dst_for(0, &dst);
+    }
+    p.bitmap = reinterpret_cast<uint32_t *>(E->d_other.p + pr + 256);
+    p.info = reinterpret_cast<TileInfo *>(E->d_other.p + pr + 256 + bitmap_bytes);
+    CK(launch_fill_u64(reinterpret_cast<uint64_t *>(p.stats), 0, 4, s));
+    CK(launch_bitmap_mark(p, s));
+    CK(launch_bitmap_greedy(p, s));
+    CK(hipMemcpyAsync(E->h_misc.p + pr, p.stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    const uint64_t kt = reinterpret_cast<const LaunchStats *>(E->h_misc.p + pr)->k_total;
+    if ((uint32_t)(kt >> kEpochShift) != 1u)
+        return (int)hipErrorUnknown;
+    const int64_t kept = (int64_t)(kt & ((1ull << kEpochShift) - 1));
+    const int64_t len = n + kept * ((int64_t)r - L);
+    uint8_t *dst = nullptr;
+    int rc = dst_for(len, &dst);
+    if (rc != 0)
+        return rc;
+    p.dst = dst;
+    p.out_len = len;
+    p.write_output = 1;
+    CK(launch_emit(p, kModeBitmap, 0, 0, groups, s));
+    *out_len = len;
+    return 0;
+}
+
+int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t *doc_lens, const int *pair_counts,
+               const uint8_t *const *pats, const int *pat_lens, const uint8_t *const *reps, const int *rep_lens,
+               const uint8_t **out_base, size_t *out_offsets, size_t *out_lens, double *stats)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    hipStream_t s = E->stream;
+    if (num_docs < 0 || !out_base || (num_docs > 0 && (!docs || !doc_lens || !pair_counts || !out_offsets ||
+                                                         !out_lens)))
+        return -1;
+    std::vector<DocState> st((size_t)num_docs);
+    int64_t total_pairs = 0;
+    int max_pairs = 0;
+    size_t total_in = 0, arena_in = 0;
+    std::vector<Piece> pieces;
+    pieces.reserve((size_t)num_docs);
+    for (int i = 0; i < num_docs; ++i) {
+        if (pair_counts[i] < 0 || (doc_lens[i] > 0 && !docs[i]))
+            return -1;
+        st[(size_t)i].pair0 = total_pairs;
+        st[(size_t)i].npairs = pair_counts[i];
+        st[(size_t)i].len = (int64_t)doc_lens[i];
+        total_pairs += pair_counts[i];
+        max_pairs = std::max(max_pairs, pair_counts[i]);
+        pieces.push_back(Piece{arena_in, docs[i], doc_lens[i]});
+        total_in += doc_lens[i];
+        arena_in += align_up(doc_lens[i], kDocAlign);
+    }
+    if (total_pairs > 0 && (!pats || !pat_lens || !reps || !rep_lens))
+        return -1;
+    for (int64_t q = 0; q < total_pairs; ++q) {
+        if (pat_lens[q] < 0 || rep_lens[q] < 0 || (pat_lens[q] > 0 && !pats[q]) || (rep_lens[q] > 0 && !reps[q]))
+            return -1;
+    }
+
+    // ---- upload every document, packed at 256-byte boundaries -------------------
+    CK(host_reserve(E->h_in, arena_in + 1));
+    copy_pieces(E->h_in.p, pieces, total_in);
+    CK(dev_reserve(E->arena[0], arena_in + 1, false, 0, s));
+    CK(dev_reserve(E->fin, align_up(arena_in + arena_in / 4, kDocAlign) + kGranule, false, 0, s));
+    if (arena_in > 0)
+        CK(hipMemcpyAsync(E->arena[0].p, E->h_in.p, arena_in, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < num_docs; ++i)
+        st[(size_t)i].cur = E->arena[0].p + pieces[(size_t)i].dst_off;
+
+    size_t fin_used = 0;
+    auto fin_slot = [&](int64_t len, uint8_t **dst, size_t *off) -> int {
+        const size_t need = fin_used + align_up((size_t)len, kDocAlign) + kDocAlign;
+        if (need > E->fin.cap) {
+            // live documents in the arenas keep their addresses; only fin moves
+            CK(dev_reserve(E->fin, need, true, fin_used, s));
+        }
+        *off = fin_used;
+        *dst = E->fin.p + fin_used;
+        fin_used += align_up((size_t)len, kDocAlign);
+        return 0;
+    };
+    // documents without pairs are final as uploaded
+    for (int i = 0; i < num_docs; ++i) {
+        DocState &d = st[(size_t)i];
+        if (d.npairs > 0)
+            continue;
+        uint8_t *dst = nullptr;
+        int rc = fin_slot(d.len, &dst, &d.fin_off);
+        if (rc != 0)
+            return rc;
+        if (d.len > 0)
+            CK(hipMemcpyAsync(dst, d.cur, (size_t)d.len, hipMemcpyDeviceToDevice, s));
+    }
+
+    CK(hipEventRecord(E->ev0, s));
+    double algo_bytes = 0;
+    std::vector<int> active;
+    for (int j = 0; j < max_pairs; ++j) {
+        active.clear();
+        for (int i = 0; i < num_docs; ++i)
+            if (st[(size_t)i].npairs > j)
+                active.push_back(i);
+        const size_t A = active.size();
+        // ---- classify, build descriptors, groups and the pattern pool ----------
+        std::vector<int> cls(A), lt(A);
+        size_t pool = 0;
+        int64_t cls_groups[kClasses + 1] = {0};
+        int64_t cls_docs[kClasses + 1] = {0};
+        for (size_t a = 0; a < A; ++a) {
+            const DocState &d = st[(size_t)active[a]];
+            const int64_t q = d.pair0 + j;
+            int t = 0;
+            const int mode = pattern_mode(pats[q], pat_lens[q], &t);
+            cls[a] = class_of(mode, t);
+            lt[a] = t;
+            pool += align_up((size_t)pat_lens[q] + 16, 16) + align_up((size_t)rep_lens[q] + 32, 16);
+            if (cls[a] < kClasses) {
+                const int64_t tiles = (d.len + kTile - 1) / kTile;
+                cls_groups[cls[a]] += (tiles + kGroupTiles - 1) / kGroupTiles;
+                cls_docs[cls[a]] += 1;
+            }
+        }
+        int64_t g_off[kClasses + 1], d_off[kClasses + 1];
+        int64_t G = 0, D = 0;
+        for (int c = 0; c < kClasses; ++c) {
+            g_off[c] = G;
+            d_off[c] = D;
+            G += cls_groups[c];
+            D += cls_docs[c];
+        }
+        const int par = j & 1;
+        CK(host_reserve(E->h_docs[par], A * sizeof(BatchDoc) + 64));
+        CK(host_reserve(E->h_groups[par], (size_t)G * sizeof(BatchGroup) + 64));
+        CK(host_reserve(E->h_cdocs[par], (size_t)D * sizeof(int32_t) + 64));
+        CK(host_reserve(E->h_pool[par], pool + 64));
+        CK(host_reserve(E->h_kept, A * sizeof(int64_t) + 64));
+        BatchDoc *hd = reinterpret_cast<BatchDoc *>(E->h_docs[par].p);
+        BatchGroup *hg = reinterpret_cast<BatchGroup *>(E->h_groups[par].p);
+        int32_t *hc = reinterpret_cast<int32_t *>(E->h_cdocs[par].p);
+        uint8_t *hpool = E->h_pool[par].p;
+        int64_t g_fill[kClasses], d_fill[kClasses];
+        for (int c = 0; c < kClasses; ++c) {
+            g_fill[c] = g_off[c];
+            d_fill[c] = d_off[c];
+        }
+        size_t pool_fill = 0;
+        for (size_t a = 0; a < A; ++a) {
+            const DocState &d = st[(size_t)active[a]];
+            const int64_t q = d.pair0 + j;
+            const int L = pat_lens[q], r = rep_lens[q];
+            BatchDoc bd{};
+            bd.src = d.cur;
+            bd.dst = nullptr;
+            bd.n = d.len;
+            bd.L = L;
+            bd.r = r;
+            pattern_head(pats[q], L, &bd.pat4, &bd.mask4);
+            bd.pat_off = (int32_t)pool_fill;
+            if (L > 0)
+                memcpy(hpool + pool_fill, pats[q], (size_t)L);
+            pool_fill += align_up((size_t)L + 16, 16);
+            bd.rep_off = (int32_t)pool_fill;
+            if (r > 0)
+                memcpy(hpool + pool_fill, reps[q], (size_t)r);
+            pool_fill += align_up((size_t)r + 32, 16);
+            const int c = cls[a];
+            if (c < kClasses) {
+                const int64_t tiles = (d.len + kTile - 1) / kTile;
+                const int64_t ng = (tiles + kGroupTiles - 1) / kGroupTiles;
+                bd.group0 = g_fill[c] - g_off[c];
+                bd.ngroups = (int32_t)ng;
+                for (int64_t g = 0; g < ng; ++g)
+                    hg[g_fill[c] + g] = BatchGroup{(int32_t)a, (int32_t)g};
+                g_fill[c] += ng;
+                hc[d_fill[c]++] = (int32_t)a;
+            }
+            hd[a] = bd;
+        }
+        CK(dev_reserve(E->d_docs, A * sizeof(BatchDoc) + 64, false, 0, s));
+        CK(dev_reserve(E->d_groups, (size_t)G * sizeof(BatchGroup) + 64, false, 0, s));
+        CK(dev_reserve(E->d_cdocs, (size_t)D * sizeof(int32_t) + 64, false, 0, s));
+        CK(dev_reserve(E->d_pool, pool + 64, false, 0, s));
+        CK(dev_reserve(E->d_gsum, (size_t)G * kMaxSummaryWords * sizeof(uint32_t) + 64, false, 0, s));
+        CK(dev_reserve(E->d_pre, (size_t)G * sizeof(ChunkState) + 64, false, 0, s));
+        CK(dev_reserve(E->d_kept, A * sizeof(int64_t) + 64, false, 0, s));
+        if (A > 0)
+            CK(hipMemcpyAsync(E->d_docs.p, hd, A * sizeof(BatchDoc), hipMemcpyHostToDevice, s));
+        if (G > 0)
+            CK(hipMemcpyAsync(E->d_groups.p, hg, (size_t)G * sizeof(BatchGroup), hipMemcpyHostToDevice, s));
+        if (D > 0)
+            CK(hipMemcpyAsync(E->d_cdocs.p, hc, (size_t)D * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        if (pool_fill > 0)
+            CK(hipMemcpyAsync(E->d_pool.p, hpool, pool_fill, hipMemcpyHostToDevice, s));
+
+        auto args_for = [&](int c) {
+            BatchArgs ba{};
+            ba.docs = reinterpret_cast<const BatchDoc *>(E->d_docs.p);
+            ba.groups = reinterpret_cast<const BatchGroup *>(E->d_groups.p) + g_off[c];
+            ba.class_docs = reinterpret_cast<const int32_t *>(E->d_cdocs.p) + d_off[c];
+            ba.pool = E->d_pool.p;
+            ba.gsum = reinterpret_cast<uint32_t *>(E->d_gsum.p) + g_off[c] * kMaxSummaryWords;
+            ba.pre = reinterpret_cast<ChunkState *>(E->d_pre.p) + g_off[c];
+            ba.doc_kept = reinterpret_cast<int64_t *>(E->d_kept.p);
+            return ba;
+        };
+        static const int kClassMode[kClasses] = {kModeBorderFree, kModeTable, kModeTable, kModeTable, kModeTable};
+        static const int kClassLt[kClasses] = {1, 2, 4, 8, 16};
+        for (int c = 0; c < kClasses; ++c) {
+            if (cls_docs[c] == 0)
+                continue;
+            const BatchArgs ba = args_for(c);
+            CK(launch_batch_count(ba, kClassMode[c], kClassLt[c], cls_groups[c], s));
+            CK(launch_batch_spine(ba, kClassMode[c], kClassLt[c], cls_docs[c], s));
+        }
+        if (A > 0)
+            CK(hipMemcpyAsync(E->h_kept.p, E->d_kept.p, A * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+
+        // ---- new lengths, destinations -----------------------------------------
+        const int nxt = (j + 1) & 1;
+        const int64_t *kept = reinterpret_cast<const int64_t *>(E->h_kept.p);
+        std::vector<int64_t> new_len(A, 0);
+        size_t arena_need = 0;
+        for (size_t a = 0; a < A; ++a) {
+            if (cls[a] >= kClasses)
+                continue;
+            const DocState &d = st[(size_t)active[a]];
+            const int64_t q = d.pair0 + j;
+            new_len[a] = d.len + kept[a] * ((int64_t)rep_lens[q] - pat_lens[q]);
+            if (d.npairs != j + 1)
+                arena_need += align_up((size_t)new_len[a], kDocAlign);
+        }
+        CK(dev_reserve(E->arena[nxt], arena_need + kDocAlign, false, 0, s));
+        // reserve the final arena for every class document finishing this round
+        // before handing out addresses, so no address below moves before its emit
+        size_t fin_need = 0;
+        for (size_t a = 0; a < A; ++a)
+            if (cls[a] < kClasses && st[(size_t)active[a]].npairs == j + 1)
+                fin_need += align_up((size_t)new_len[a], kDocAlign);
+        if (fin_used + fin_need + kDocAlign > E->fin.cap)
+            CK(dev_reserve(E->fin, fin_used + fin_need + kDocAlign, true, fin_used, s));
+        size_t arena_used = 0;
+        for (size_t a = 0; a < A; ++a) {
+            if (cls[a] >= kClasses)
+                continue;
+            DocState &d = st[(size_t)active[a]];
+            uint8_t *dst;
+            if (d.npairs == j + 1) {
+                int rc = fin_slot(new_len[a], &dst, &d.fin_off);
+                if (rc != 0)
+                    return rc;
+            } else {
+                dst = E->arena[nxt].p + arena_used;
+                arena_used += align_up((size_t)new_len[a], kDocAlign);
+            }
+            hd[a].dst = dst;
+        }
+        if (A > 0)
+            CK(hipMemcpyAsync(E->d_docs.p, hd, A * sizeof(BatchDoc), hipMemcpyHostToDevice, s));
+        for (int c = 0; c < kClasses; ++c) {
+            if (cls_docs[c] == 0)
+                continue;
+            CK(launch_batch_emit(args_for(c), kClassMode[c], kClassLt[c], cls_groups[c], s));
+        }
+        // ---- empty-pattern and bitmap-path documents, one at a time ------------
+        // (after a sync: the loop edits descriptors the upload above reads)
+        if (std::find_if(cls.begin(), cls.end(), [](int c) { return c >= kClasses; }) != cls.end())
+            CK(hipStreamSynchronize(s));
+        for (size_t a = 0; a < A; ++a) {
+            if (cls[a] < kClasses)
+                continue;
+            DocState &d = st[(size_t)active[a]];
+            const int64_t q = d.pair0 + j;
+            const bool last = d.npairs == j + 1;
+            auto dst_for = [&](int64_t len, uint8_t **dst) -> int {
+                if (last)
+                    return fin_slot(len, dst, &d.fin_off);
+                const size_t need = arena_used + align_up((size_t)len, kDocAlign) + kDocAlign;
+                if (need > E->arena[nxt].cap) {
+                    // arena[nxt] already holds this round's outputs: move them (queued
+                    // after their emits) and re-point their documents
+                    DevBuf old = E->arena[nxt];
+                    E->arena[nxt] = DevBuf{};
+                    hipError_t e = dev_reserve(E->arena[nxt], need, false, 0, s);
+                    if (e != hipSuccess) {
+                        E->arena[nxt] = old;
+                        return (int)e;
+                    }
+                    if (arena_used > 0) {
+                        e = hipMemcpyAsync(E->arena[nxt].p, old.p, arena_used, hipMemcpyDeviceToDevice, s);
+                        if (e != hipSuccess)
+                            return (int)e;
+                    }
+                    for (size_t b = 0; b < A; ++b)
+                        if (hd[b].dst >= old.p && hd[b].dst < old.p + old.cap)
+                            hd[b].dst = E->arena[nxt].p + (hd[b].dst - old.p);
+                    e = hipStreamSynchronize(s);
+                    (void)hipFree(old.p);
+                    if (e != hipSuccess)
+                        return (int)e;
+                }
+                *dst = E->arena[nxt].p + arena_used;
+                arena_used += align_up((size_t)len, kDocAlign);
+                return 0;
+            };
+            int64_t len = 0;
+            int rc = run_other(E, d, pats[q], pat_lens[q], reps[q], rep_lens[q], &len, dst_for);
+            if (rc != 0)
+                return rc;
+            new_len[a] = len;
+            // record where the output went (dst_for may have moved other documents)
+            hd[a].dst = last ? E->fin.p + d.fin_off : E->arena[nxt].p + (arena_used - align_up((size_t)len, kDocAlign));
+        }
+        for (size_t a = 0; a < A; ++a) {
+            DocState &d = st[(size_t)active[a]];
+            algo_bytes += (double)d.len + (double)new_len[a];
+            d.len = new_len[a];
+            d.cur = d.npairs == j + 1 ? E->fin.p + d.fin_off : hd[a].dst;
+        }
+    }
+    CK(hipEventRecord(E->ev1, s));
+
+    // ---- results ----------------------------------------------------------------
+    CK(host_reserve(E->h_out, fin_used + 64));
+    if (fin_used > 0)
+        CK(hipMemcpyAsync(E->h_out.p, E->fin.p, fin_used, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    for (int i = 0; i < num_docs; ++i) {
+        out_offsets[i] = st[(size_t)i].fin_off;
+        out_lens[i] = (size_t)st[(size_t)i].len;
+    }
+    *out_base = E->h_out.p;
+    if (stats) {
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+        stats[0] = ms;
+        stats[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        stats[2] = algo_bytes;
+        stats[3] = max_pairs;
+    }
+    return 0;
+}
+
+void engine_free(Engine *E)
+{
+    DevBuf *dev[] = {&E->arena[0], &E->arena[1], &E->fin, &E->d_docs, &E->d_groups, &E->d_cdocs,
+                     &E->d_pool, &E->d_gsum, &E->d_pre, &E->d_kept, &E->d_other};
+    for (DevBuf *b : dev)
+        if (b->p)
+            (void)hipFree(b->p);
+    HostBuf *host[] = {&E->h_in, &E->h_docs[0], &E->h_docs[1], &E->h_groups[0], &E->h_groups[1],
+                       &E->h_cdocs[0], &E->h_cdocs[1], &E->h_pool[0], &E->h_pool[1], &E->h_kept,
+                       &E->h_out, &E->h_misc};
+    for (HostBuf *b : host)
+        if (b->p)
+            (void)hipHostFree(b->p);
+    if (E->ev0)
+        (void)hipEventDestroy(E->ev0);
+    if (E->ev1)
+        (void)hipEventDestroy(E->ev1);
+    if (E->stream)
+        (void)hipStreamDestroy(E->stream);
+}
+
+} // namespace
+
+extern "C" {
+
+CR_API int cr_batch_create(void **pb)
+{
+    if (!pb)
+        return -1;
+    *pb = nullptr;
+    Engine *E = new (std::nothrow) Engine();
+    if (!E)
+        return -2;
+    hipError_t e = hipGetDevice(&E->device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&E->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&E->ev1);
+    if (e != hipSuccess) {
+        engine_free(E);
+        delete E;
+        return (int)e;
+    }
+    *pb = E;
+    return 0;
+}
+
+CR_API int cr_batch_run(void *b, int num_docs, const uint8_t *const *docs, const size_t *doc_lens,
+                        const int *pair_counts, const uint8_t *const *pats, const int *pat_lens,
+                        const uint8_t *const *reps, const int *rep_lens, const uint8_t **out_base,
+                        size_t *out_offsets, size_t *out_lens, double *stats)
+{
+    if (!b)
+        return -1;
+    Engine *E = static_cast<Engine *>(b);
+    std::lock_guard<std::mutex> lock(E->mtx);
+    DeviceGuard dg(E->device);
+    return engine_run(E, num_docs, docs, doc_lens, pair_counts, pats, pat_lens, reps, rep_lens, out_base,
+                      out_offsets, out_lens, stats);
+}
+
+CR_API void cr_batch_destroy(void *b)
+{
+    if (!b)
+        return;
+    Engine *E = static_cast<Engine *>(b);
+    {
+        std::lock_guard<std::mutex> lock(E->mtx);
+        DeviceGuard dg(E->device);
+        if (E->stream)
+            (void)hipStreamSynchronize(E->stream);
+        engine_free(E);
+    }
+    delete E;
+}
+
+} // extern "C"

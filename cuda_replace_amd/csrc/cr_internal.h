@@ -88,6 +88,44 @@ struct ReplaceParams {
     int64_t num_groups;
 };
 
+// ---- batches of independent documents ----------------------------------------
+// One document of a batch round: its current bytes, its destination and the pair
+// the round applies to it.
+struct BatchDoc {
+    const uint8_t *src;    // device, 16-byte aligned
+    uint8_t *dst;          // device, 16-byte aligned (set once the round's sizes are known)
+    int64_t n;             // current length
+    int64_t group0;        // first of the document's groups in its class's group list
+    int32_t ngroups;
+    int32_t L;
+    int32_t r;
+    uint32_t pat4;
+    uint32_t mask4;
+    int32_t pat_off;       // pattern bytes at pool + pat_off, replacement at pool + rep_off
+    int32_t rep_off;
+    int32_t pad;
+};
+
+// One workgroup of a batched count or emit launch: a group of one document.
+struct BatchGroup {
+    int32_t doc;           // index into the round's BatchDoc array
+    int32_t g;             // group index within the document
+};
+
+struct BatchArgs {
+    const BatchDoc *docs;
+    const BatchGroup *groups;   // this class's groups, document by document
+    const int32_t *class_docs;  // this class's documents (spine: one wave each)
+    const uint8_t *pool;        // patterns and replacements of the round
+    uint32_t *gsum;             // one summary per group of the class
+    ChunkState *pre;            // one state per group of the class
+    int64_t *doc_kept;          // kept starts per document of the round
+};
+
+hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s);
+hipError_t launch_batch_spine(const BatchArgs &a, int mode, int table_size, int64_t docs, hipStream_t s);
+hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s);
+
 // ---- launchers (cr_kernels.hip) --------------------------------------------
 // One chunk of groups [first_group, first_group + groups): count, spine, emit.
 hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int64_t first_group,

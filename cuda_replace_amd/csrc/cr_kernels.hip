@@ -401,8 +401,7 @@ __device__ __forceinline__ Table<LT> segment_table(uint32_t m, int L)
 // table (carry in -> carry out, kept), composed tile by tile in position order.
 // Workgroups never wait on each other.
 template <int MODE, int LT>
-__global__ void __launch_bounds__(kThreads)
-count_kernel(ReplaceParams p, int64_t first_group)
+__device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, uint32_t *summary)
 {
     __shared__ __align__(16) uint8_t s_in[kInBytes];
     __shared__ uint8_t s_pat[kMaxSinglePassPat];
@@ -412,7 +411,6 @@ count_kernel(ReplaceParams p, int64_t first_group)
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
-    const int64_t g = first_group + blockIdx.x;
     const int64_t t0 = g * kGroupTiles;
     if (t0 >= p.num_tiles)
         return;
@@ -436,7 +434,7 @@ count_kernel(ReplaceParams p, int64_t first_group)
             s_red[wave] = cnt;
         __syncthreads();
         if (tid == 0)
-            p.gsum[g] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+            summary[0] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
     } else {
         Table<LT> grp = tidentity<LT>();
         for (int i = 0; i < kGroupTiles; ++i) {
@@ -462,8 +460,17 @@ count_kernel(ReplaceParams p, int64_t first_group)
             __syncthreads();
         }
         if (tid < LT)
-            p.gsum[g * LT + tid] = tsel<LT>(grp, (uint32_t)tid);
+            summary[tid] = tsel<LT>(grp, (uint32_t)tid);
     }
+}
+
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kThreads)
+count_kernel(ReplaceParams p, int64_t first_group)
+{
+    constexpr int W = MODE == kModeTable ? LT : 1;
+    const int64_t g = first_group + blockIdx.x;
+    count_group<MODE, LT>(p, g, p.gsum + g * W);
 }
 
 // ---- spine: exclusive kept state of every group of a chunk -------------------
@@ -641,9 +648,10 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
 // The group's kept prefix and carry come from the spine's record for the group.
 // The bitmap mode instead reads each tile's prefix and carry from the records of
 // the serial greedy, and the kept starts from the bitmap.
+// g: the group within p's buffer; pre: its state (kept before it, carry into it),
+// unused in bitmap mode; per-tile records are written when p.info is set.
 template <int MODE, int LT>
-__global__ void __launch_bounds__(kThreads)
-emit_kernel(ReplaceParams p, int64_t first_group)
+__device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, const ChunkState *pre)
 {
     __shared__ __align__(16) uint8_t s_in[kInBytes];
     __shared__ __align__(16) uint8_t s_out[kStage];
@@ -662,7 +670,6 @@ emit_kernel(ReplaceParams p, int64_t first_group)
     const int64_t n = p.n;
     const int64_t nt = p.num_tiles;
     const int64_t diff = (int64_t)r - (int64_t)L;
-    const int64_t g = first_group + blockIdx.x;
     const int64_t t0 = g * kGroupTiles;
     if (t0 >= nt)
         return;
@@ -683,9 +690,9 @@ emit_kernel(ReplaceParams p, int64_t first_group)
     int64_t k_run = 0;   // kept starts before the current tile
     int c_run = 0;       // carry into the current tile
     if (MODE != kModeBitmap) {
-        const ChunkState pre = p.chunks_pre[g];
-        k_run = pre.k_base;
-        c_run = pre.carry;
+        const ChunkState st = *pre;
+        k_run = st.k_base;
+        c_run = st.carry;
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -848,7 +855,7 @@ emit_kernel(ReplaceParams p, int64_t first_group)
         }
 
         // ---- per-tile record (prefix queries) ------------------------------------
-        if (tid == 0 && MODE != kModeBitmap) {
+        if (tid == 0 && MODE != kModeBitmap && p.info) {
             TileInfo ti_rec;
             ti_rec.k_excl = k_run;
             ti_rec.last_abs = tile_k > 0 ? start + tile_last : -1;
@@ -923,6 +930,129 @@ emit_kernel(ReplaceParams p, int64_t first_group)
         c_run = c_next;
         __syncthreads();   // s_out, s_wsum and s_wtab are reused by the next tile
     }
+}
+
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kThreads)
+emit_kernel(ReplaceParams p, int64_t first_group)
+{
+    const int64_t g = first_group + blockIdx.x;
+    emit_group<MODE, LT>(p, g, p.chunks_pre + g);
+}
+
+// ---- batches of independent documents ---------------------------------------
+// The count and emit launches of a batch round run the same per-group code as a
+// single buffer, on a per-document view built from the round's BatchDoc.
+__device__ __forceinline__ ReplaceParams batch_view(const BatchArgs &a, int doc)
+{
+    const BatchDoc d = a.docs[doc];
+    ReplaceParams p{};
+    p.src = d.src;
+    p.dst = d.dst;
+    p.n = d.n;
+    p.floor = 0;
+    p.out_len = -1;
+    p.pat = a.pool + d.pat_off;
+    p.rep = a.pool + d.rep_off;
+    p.L = d.L;
+    p.r = d.r;
+    p.pat4 = d.pat4;
+    p.mask4 = d.mask4;
+    p.num_tiles = (d.n + kTile - 1) / kTile;
+    p.num_groups = d.ngroups;
+    p.write_output = 1;
+    return p;
+}
+
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kThreads)
+batch_count_kernel(BatchArgs a)
+{
+    constexpr int W = MODE == kModeTable ? LT : 1;
+    const BatchGroup bg = a.groups[blockIdx.x];
+    const ReplaceParams p = batch_view(a, bg.doc);
+    count_group<MODE, LT>(p, bg.g, a.gsum + (int64_t)blockIdx.x * W);
+}
+
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kThreads)
+batch_emit_kernel(BatchArgs a)
+{
+    const BatchGroup bg = a.groups[blockIdx.x];
+    const ReplaceParams p = batch_view(a, bg.doc);
+    emit_group<MODE, LT>(p, bg.g, a.pre + blockIdx.x);
+}
+
+// One wave per document: an ordered scan of the document's group summaries, 64
+// groups per step, starting from no kept start and carry 0.  Writes every group's
+// exclusive state and the document's kept total.
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kWave)
+batch_spine_kernel(BatchArgs a)
+{
+    const int lane = threadIdx.x;
+    const int doc = a.class_docs[blockIdx.x];
+    const BatchDoc d = a.docs[doc];
+    int64_t k = 0;
+    uint32_t carry = 0;
+    for (int64_t b = 0; b < d.ngroups; b += kWave) {
+        const int64_t gi = b + lane;
+        const int64_t slot = d.group0 + gi;
+        if constexpr (MODE != kModeTable) {
+            const uint32_t v = gi < d.ngroups ? a.gsum[slot] : 0u;
+            uint32_t inc = v;
+#pragma unroll
+            for (int dd = 1; dd < kWave; dd <<= 1) {
+                const uint32_t u = __shfl_up(inc, dd, kWave);
+                if (lane >= dd)
+                    inc += u;
+            }
+            if (gi < d.ngroups) {
+                ChunkState cs;
+                cs.k_base = k + inc - v;
+                cs.carry = 0;
+                cs.pad = 0;
+                a.pre[slot] = cs;
+            }
+            k += __shfl(inc, kWave - 1, kWave);
+        } else {
+            Table<LT> t = tidentity<LT>();
+            if (gi < d.ngroups) {
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    t.e[c] = a.gsum[slot * LT + c];
+            }
+            Table64<LT> inc = widen<LT>(t);
+#pragma unroll
+            for (int dd = 1; dd < kWave; dd <<= 1) {
+                Table64<LT> nb;
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    nb.e[c] = __shfl_up(inc.e[c], dd, kWave);
+                if (lane >= dd)
+                    inc = tcompose64<LT>(nb, inc);
+            }
+            uint64_t e = (uint64_t)carry << kCarryShift;   // lane 0: nothing before it
+#pragma unroll
+            for (int c = 0; c < LT; ++c) {
+                const uint64_t u = __shfl_up(inc.e[c], 1, kWave);
+                if (lane > 0 && (uint32_t)c == carry)
+                    e = u;
+            }
+            if (gi < d.ngroups) {
+                ChunkState cs;
+                cs.k_base = k + (int64_t)(e & kCountMask);
+                cs.carry = (int)(e >> kCarryShift);
+                cs.pad = 0;
+                a.pre[slot] = cs;
+            }
+            const uint64_t el = tsel64<LT>(wave_broadcast_last<LT>(inc), carry);
+            k += (int64_t)(el & kCountMask);
+            carry = (uint32_t)(el >> kCarryShift);
+        }
+    }
+    if (lane == 0)
+        a.doc_kept[doc] = k;
 }
 
 // ---- bitmap mode: candidate bitmap -------------------------------------------
@@ -1350,6 +1480,56 @@ hipError_t launch_presence_index(const uint8_t *src, int64_t n, int64_t chunk, u
     hipLaunchKernelGGL(presence_kernel, dim3((unsigned)num_chunks), dim3(kThreads), 0, s, src, n, chunk,
                        masks);
     return hipGetLastError();
+}
+
+// ---- batch launchers ----------------------------------------------------------
+template <int MODE, int LT>
+hipError_t batch_t(const BatchArgs &a, int which, int64_t n, hipStream_t s)
+{
+    if (which == 0)
+        hipLaunchKernelGGL((batch_count_kernel<MODE, LT>), dim3((unsigned)n), dim3(kThreads), 0, s, a);
+    else if (which == 1)
+        hipLaunchKernelGGL((batch_spine_kernel<MODE, LT>), dim3((unsigned)n), dim3(kWave), 0, s, a);
+    else
+        hipLaunchKernelGGL((batch_emit_kernel<MODE, LT>), dim3((unsigned)n), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t batch_dispatch(const BatchArgs &a, int mode, int table_size, int which, int64_t n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    if (mode == kModeBorderFree)
+        return batch_t<kModeBorderFree, 1>(a, which, n, s);
+    if (mode != kModeTable)
+        return hipErrorInvalidValue;
+    switch (table_size) {
+    case 2:
+        return batch_t<kModeTable, 2>(a, which, n, s);
+    case 4:
+        return batch_t<kModeTable, 4>(a, which, n, s);
+    case 8:
+        return batch_t<kModeTable, 8>(a, which, n, s);
+    case 16:
+        return batch_t<kModeTable, 16>(a, which, n, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s)
+{
+    return batch_dispatch(a, mode, table_size, 0, groups, s);
+}
+
+hipError_t launch_batch_spine(const BatchArgs &a, int mode, int table_size, int64_t docs, hipStream_t s)
+{
+    return batch_dispatch(a, mode, table_size, 1, docs, s);
+}
+
+hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s)
+{
+    return batch_dispatch(a, mode, table_size, 2, groups, s);
 }
 
 } // namespace cr

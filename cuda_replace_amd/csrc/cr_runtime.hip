@@ -11,6 +11,7 @@
 // 997-1140 ensure_* allocators, 2097-2159 apply_once, 2161-2603 exports) with
 // 64-bit lengths and no per-call device allocation of pattern buffers.
 #include "cr_internal.h"
+#include "cr_pattern.h"
 #include "../../include/cuda_replace.h"
 
 #include <limits.h>
@@ -26,7 +27,6 @@ using namespace cr;
 namespace {
 
 constexpr size_t kAllocGranule = 2u << 20;   // device buffers grow in 2 MiB steps
-constexpr int kModeEmpty = 4;                // empty pattern (bytes.replace inserts)
 
 struct Session {
     std::mutex mtx;
@@ -150,42 +150,6 @@ hipError_t grow_host(uint8_t **ptr, size_t *cap, size_t need)
     *ptr = (uint8_t *)np;
     *cap = nc;
     return hipSuccess;
-}
-
-// Pattern is border-free iff its longest proper border (KMP failure value) is 0;
-// then two occurrences can never overlap.
-bool border_free(const uint8_t *pat, int L)
-{
-    if (L <= 1)
-        return true;
-    std::vector<int> f((size_t)L, 0);
-    int k = 0;
-    for (int i = 1; i < L; ++i) {
-        while (k > 0 && pat[i] != pat[k])
-            k = f[(size_t)k - 1];
-        if (pat[i] == pat[k])
-            ++k;
-        f[(size_t)i] = k;
-    }
-    return f[(size_t)L - 1] == 0;
-}
-
-int pattern_mode(const uint8_t *pat, int L, int *table_size)
-{
-    if (L == 0)
-        return kModeEmpty;
-    const bool bf = border_free(pat, L);
-    if (bf && L <= kMaxSinglePassPat)
-        return kModeBorderFree;
-    if (!bf && L <= kMaxTablePat) {
-        int lt = 2;
-        while (lt < L)
-            lt <<= 1;
-        if (table_size)
-            *table_size = lt;
-        return kModeTable;
-    }
-    return kModeBitmap;
 }
 
 hipError_t ensure_misc(Session *s)

@@ -14,6 +14,12 @@ Workloads (sizes are the full benchmark sizes; tests pass smaller ``n``):
 * ``text_fragment`` -- the 4-byte fragment of a frequent vocabulary word whose
   density in a 4 MiB sample is closest to 1%; the expand replacement is 12
   seeded uppercase bytes, the shrink replacement one byte.
+* ``uneven_documents`` -- 8192 documents of the text with log-normal sizes
+  (median 64 KiB, sigma 1.5, clipped to 1 KiB - 32 MiB), each with 1-16 pairs
+  drawn from ``pair_pool``: patterns are 1-16 byte slices of the text at seeded
+  positions (so they occur, some self-overlapping), replacements 0-32 seeded
+  bytes of ``A-Z0-9``; a pair whose replacement would grow the sample by more
+  than 5% gets a replacement no longer than its pattern.
 """
 from __future__ import annotations
 
@@ -21,6 +27,7 @@ import functools
 
 import numpy as np
 
+KiB = 1 << 10
 MiB = 1 << 20
 GiB = 1 << 30
 
@@ -108,3 +115,38 @@ def text_config(n: int, seed: int = 2):
     text = zipf_text(n, seed)
     frag = text_fragment(text[: 4 * MiB], seed)
     return text, frag, expand_replacement(seed), b"#"
+
+
+_REP_ALPHABET = np.frombuffer(b"ABCDEFGHIJKLMNOPQRSTUVWXYZ0123456789", dtype=np.uint8)
+
+
+def pair_pool(sample: bytes, count: int = 256, seed: int = 4, max_growth: float = 0.05):
+    """Seeded (pattern, replacement) pool over ``sample`` (see module docstring)."""
+    rng = np.random.default_rng([seed, 0xBA7C])
+    pool = []
+    while len(pool) < count:
+        L = int(rng.integers(1, 17))
+        pos = int(rng.integers(0, len(sample) - L))
+        pat = sample[pos:pos + L]
+        r = int(rng.integers(0, 33))
+        density = sample.count(pat) / max(1, len(sample))
+        if density * max(0, r - L) > max_growth:
+            r = int(rng.integers(0, L + 1))
+        rep = _REP_ALPHABET[rng.integers(0, len(_REP_ALPHABET), size=r)].tobytes()
+        pool.append((pat, rep))
+    return pool
+
+
+def uneven_documents(num_docs: int = 8192, median: int = 64 * KiB, sigma: float = 1.5, lo: int = KiB,
+                     hi: int = 32 * MiB, seed: int = 4, text_seed: int = 2, max_pairs: int = 16,
+                     pool_size: int = 256):
+    """(docs, pairs): the batched-documents workload (see module docstring)."""
+    rng = np.random.default_rng([seed, 0xD0C5])
+    sizes = np.clip(np.rint(rng.lognormal(np.log(median), sigma, num_docs)), lo, hi).astype(np.int64)
+    text = zipf_text(int(sizes.sum()), text_seed)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    docs = [text[int(offs[i]):int(offs[i + 1])] for i in range(num_docs)]
+    pool = pair_pool(text[:4 * MiB], pool_size, seed)
+    counts = rng.integers(1, max_pairs + 1, size=num_docs)
+    pairs = [[pool[int(k)] for k in rng.integers(0, pool_size, size=int(c))] for c in counts]
+    return docs, pairs

@@ -42,6 +42,7 @@ REQUIRED_SYMBOLS = (
     "cuda_replace_query_prefix", "cuda_replace_apply_batch", "cuda_replace_result",
     "cuda_replace_close", "cuda_replace_version",
     "cr_session_length", "cr_query_prefix64", "cr_replace_device", "cr_pattern_mode",
+    "cr_batch_create", "cr_batch_run", "cr_batch_destroy",
 )
 
 
@@ -104,6 +105,14 @@ class CudaReplaceLib:
         lib.cr_replace_device.restype = C.c_int
         lib.cr_pattern_mode.argtypes = [c_uint8_p, C.c_int]
         lib.cr_pattern_mode.restype = C.c_int
+        lib.cr_batch_create.argtypes = [C.POINTER(c_void_p)]
+        lib.cr_batch_create.restype = C.c_int
+        lib.cr_batch_run.argtypes = [c_void_p, C.c_int, c_void_p, c_size_t_p, c_int_p, c_void_p, c_int_p,
+                                     c_void_p, c_int_p, C.POINTER(c_uint8_p), c_size_t_p, c_size_t_p,
+                                     C.POINTER(C.c_double)]
+        lib.cr_batch_run.restype = C.c_int
+        lib.cr_batch_destroy.argtypes = [c_void_p]
+        lib.cr_batch_destroy.restype = None
 
     def version(self) -> str:
         return self.lib.cuda_replace_version().decode()
@@ -276,3 +285,61 @@ def gpu_replace_streaming(lib: CudaReplaceLib, src: bytes, pairs: Iterable[Tuple
                 pos += cut
             out = b"".join(pieces)
     return out
+
+
+class BatchEngine:
+    """Independent documents, each with its own ordered list of (pattern,
+    replacement) pairs, replaced together on the GPU (``cr_batch_run``).  Every
+    result equals applying the document's pairs in order with ``bytes.replace``.
+    The engine keeps its device arenas and pinned staging between runs."""
+
+    def __init__(self, lib: CudaReplaceLib):
+        self.lib = lib
+        self.h = c_void_p()
+        rc = lib.lib.cr_batch_create(C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError(f"cr_batch_create failed rc={rc}")
+        self.last_stats = None
+
+    def run(self, docs: Sequence[bytes], pairs: Sequence[Sequence[Tuple[bytes, bytes]]]) -> List[bytes]:
+        if len(docs) != len(pairs):
+            raise ValueError("one pair list per document")
+        n = len(docs)
+        flat = [pr for per in pairs for pr in per]
+        m = len(flat)
+        doc_ptrs = (c_uint8_p * max(n, 1))(*[_u8(d) for d in docs])
+        doc_lens = (c_size_t * max(n, 1))(*[len(d) for d in docs])
+        counts = (C.c_int * max(n, 1))(*[len(per) for per in pairs])
+        pat_ptrs = (c_uint8_p * max(m, 1))(*[_u8(p) for p, _ in flat])
+        pat_lens = (C.c_int * max(m, 1))(*[len(p) for p, _ in flat])
+        rep_ptrs = (c_uint8_p * max(m, 1))(*[_u8(r) for _, r in flat])
+        rep_lens = (C.c_int * max(m, 1))(*[len(r) for _, r in flat])
+        base = c_uint8_p()
+        offs = (c_size_t * max(n, 1))()
+        lens = (c_size_t * max(n, 1))()
+        stats = (C.c_double * 4)()
+        rc = self.lib.lib.cr_batch_run(self.h, n, doc_ptrs, doc_lens, counts, pat_ptrs, pat_lens, rep_ptrs,
+                                       rep_lens, C.byref(base), offs, lens, stats)
+        if rc != 0:
+            raise RuntimeError(f"cr_batch_run failed rc={rc}")
+        self.last_stats = {"device_ms": stats[0], "wall_ms": stats[1], "algo_bytes": stats[2],
+                           "rounds": int(stats[3])}
+        addr = C.cast(base, c_void_p).value or 0
+        return [C.string_at(addr + offs[i], lens[i]) if lens[i] else b"" for i in range(n)]
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.lib.cr_batch_destroy(self.h)
+            self.h = c_void_p()
+
+    def __enter__(self) -> "BatchEngine":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -126,6 +126,32 @@ CR_API int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d
  * tables (self-overlapping, length <= 16), 3 bitmap + serial greedy, 0 empty. */
 CR_API int cr_pattern_mode(const uint8_t *pat, int pat_len);
 
+/* ---- batched documents (no counterpart in the reference; its callers loop over
+ * cuda_replace_unified per document and pair, cuda_replace.cu:2579-2601) ----- */
+
+/* Creates a batch engine on the current device: its own stream, plus device
+ * arenas and pinned staging that stay resident between runs. */
+CR_API int cr_batch_create(void **pb);
+
+/* Runs num_docs independent documents.  Document i (docs[i], doc_lens[i] bytes)
+ * gets pair_counts[i] (pattern, replacement) pairs applied in order; the pairs are
+ * listed document after document in pats / pat_lens / reps / rep_lens.  Each
+ * result equals applying that document's pairs one after another with
+ * bytes.replace, including empty patterns.  Results are packed into an
+ * engine-owned pinned buffer *out_base, valid until the next run or destroy;
+ * document i's result is out_base[out_offsets[i] .. out_offsets[i] + out_lens[i]).
+ * stats (optional, 4 doubles): device ms of the rounds (host round trips included,
+ * uploads of the documents and the final download excluded), wall ms of the call,
+ * algorithmic bytes (input + output of every applied pair), number of rounds.
+ * Calls on one engine serialize. */
+CR_API int cr_batch_run(void *b, int num_docs, const uint8_t *const *docs, const size_t *doc_lens,
+                        const int *pair_counts, const uint8_t *const *pats, const int *pat_lens,
+                        const uint8_t *const *reps, const int *rep_lens, const uint8_t **out_base,
+                        size_t *out_offsets, size_t *out_lens, double *stats);
+
+/* Destroys a batch engine; NULL is ignored. */
+CR_API void cr_batch_destroy(void *b);
+
 #ifdef __cplusplus
 }
 #endif
