@@ -15,6 +15,11 @@ roofline = algorithmic bytes of a border-free pass (n_in + n_out) over the pass'
            comes from the committed rocprofv3 summary (profiles/traffic.json).
 cpu_baseline = bytes.replace, single thread, on the same three inputs (one pass
            each, best of two), plus a fork-based per-core chunked variant.
+batched  = the uneven-documents workload (8192 documents, log-normal sizes, 1-16
+           pairs each) through the batch engine, host to host, every result checked
+           against the oracle; beside it the per-call loop of the reference's callers
+           (one cuda_replace_unified per document and pair) timed on every 32nd
+           document.  Reported as an extra field of the same line.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--size-mib 1024]
 """
@@ -84,6 +89,55 @@ def cpu_parallel(key: str, pat: bytes, rep: bytes, procs: int):
 
 
 # ------------------------------------------------------------------ main
+def batch_bench(lib, ro, num_docs: int) -> dict:
+    """Batched uneven documents vs the per-call loop (see module docstring)."""
+    from cuda_replace_amd import BatchEngine, workloads
+    t0 = time.time()
+    docs, pairs = workloads.uneven_documents(num_docs=num_docs)
+    total = sum(len(d) for d in docs)
+    log(f"[bench] batch workload: {num_docs} docs, {total / 2**30:.3f} GiB, "
+        f"{sum(len(p) for p in pairs)} pairs, generated in {time.time() - t0:.1f}s")
+    eng = BatchEngine(lib)
+    eng.run(docs[:256], pairs[:256])                    # warm the engine's buffers
+    eng.run(docs, pairs)
+    t = time.perf_counter()
+    out = eng.run(docs, pairs, copy=False)      # results as views of the pinned output
+    wall = time.perf_counter() - t
+    st = dict(eng.last_stats)
+    t = time.perf_counter()
+    exact = all(out[i] == ro.apply_batch(docs[i], pairs[i]) for i in range(num_docs))
+    del out
+    eng.close()
+    log(f"[bench] batch: {wall * 1e3:.1f} ms host-to-host, bit_exact={exact} "
+        f"(oracle {time.perf_counter() - t:.1f}s)")
+    sample = list(range(0, num_docs, 32))
+    t = time.perf_counter()
+    for i in sample:
+        d = docs[i]
+        for pat, rep in pairs[i]:
+            d = lib.unified(d, pat, rep)
+    per_call = time.perf_counter() - t
+    docs_s = num_docs / wall
+    per_call_docs_s = len(sample) / per_call
+    log(f"[bench] per-call loop: {len(sample)} docs in {per_call:.2f}s")
+    return {
+        "workload": f"{num_docs} documents, log-normal sizes (median 64 KiB, sigma 1.5, 1 KiB-32 MiB), "
+                    f"1-16 pairs each (pattern 1-16 B, rep 0-32 B)",
+        "input_bytes": total,
+        "docs_per_s": round(docs_s, 1),
+        "per_call_docs_per_s": round(per_call_docs_s, 1),
+        "speedup": round(docs_s / per_call_docs_s, 2),
+        "host_to_host_ms": round(wall * 1e3, 2),
+        "engine_wall_ms": round(st["wall_ms"], 2),
+        "device_ms": round(st["device_ms"], 3),
+        "device_algo_GBps": round(st["algo_bytes"] / (st["device_ms"] / 1e3) / 1e9, 1),
+        "rounds": st["rounds"],
+        "per_call_sample": f"every 32nd document ({len(sample)} docs), one cuda_replace_unified "
+                           f"per document and pair",
+        "bit_exact": exact,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +145,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size-mib", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline timing")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-documents measurement")
+    ap.add_argument("--batch-docs", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the replace path runs on a single MI355X "
@@ -203,6 +259,7 @@ def main() -> int:
         exact &= ok
         log(f"[bench] {name}: out_len={ol} bit_exact={ok}")
     lib.lib.cuda_replace_close(h)
+    batched = None if args.no_batch else batch_bench(lib, ro, args.batch_docs)
 
     total_in = 3 * n * args.steps
     value = total_in / wall / 1e9
@@ -261,10 +318,11 @@ def main() -> int:
                                       if par else None),
                          "cpu": platform.processor() or platform.machine()},
         "device_copy_GBps": round(copy_gbs, 1),
+        "batched": batched,
         "detail": detail,
     }
     print(json.dumps(line), flush=True)
-    return 0 if exact else 1
+    return 0 if exact and (batched is None or batched["bit_exact"]) else 1
 
 
 if __name__ == "__main__":

@@ -107,8 +107,9 @@ class CudaReplaceLib:
         lib.cr_pattern_mode.restype = C.c_int
         lib.cr_batch_create.argtypes = [C.POINTER(c_void_p)]
         lib.cr_batch_create.restype = C.c_int
-        lib.cr_batch_run.argtypes = [c_void_p, C.c_int, c_void_p, c_size_t_p, c_int_p, c_void_p, c_int_p,
-                                     c_void_p, c_int_p, C.POINTER(c_uint8_p), c_size_t_p, c_size_t_p,
+        c_char_pp = C.POINTER(C.c_char_p)
+        lib.cr_batch_run.argtypes = [c_void_p, C.c_int, c_char_pp, c_size_t_p, c_int_p, c_char_pp, c_int_p,
+                                     c_char_pp, c_int_p, C.POINTER(c_uint8_p), c_size_t_p, c_size_t_p,
                                      C.POINTER(C.c_double)]
         lib.cr_batch_run.restype = C.c_int
         lib.cr_batch_destroy.argtypes = [c_void_p]
@@ -301,19 +302,24 @@ class BatchEngine:
             raise RuntimeError(f"cr_batch_create failed rc={rc}")
         self.last_stats = None
 
-    def run(self, docs: Sequence[bytes], pairs: Sequence[Sequence[Tuple[bytes, bytes]]]) -> List[bytes]:
+    def run(self, docs: Sequence[bytes], pairs: Sequence[Sequence[Tuple[bytes, bytes]]],
+            copy: bool = True) -> list:
+        """Results in document order: ``bytes`` objects, or with ``copy=False``
+        read-only memoryviews into the engine's pinned output buffer, valid until
+        the next ``run`` or ``close``."""
         if len(docs) != len(pairs):
             raise ValueError("one pair list per document")
         n = len(docs)
-        flat = [pr for per in pairs for pr in per]
-        m = len(flat)
-        doc_ptrs = (c_uint8_p * max(n, 1))(*[_u8(d) for d in docs])
-        doc_lens = (c_size_t * max(n, 1))(*[len(d) for d in docs])
-        counts = (C.c_int * max(n, 1))(*[len(per) for per in pairs])
-        pat_ptrs = (c_uint8_p * max(m, 1))(*[_u8(p) for p, _ in flat])
-        pat_lens = (C.c_int * max(m, 1))(*[len(p) for p, _ in flat])
-        rep_ptrs = (c_uint8_p * max(m, 1))(*[_u8(r) for _, r in flat])
-        rep_lens = (C.c_int * max(m, 1))(*[len(r) for _, r in flat])
+        pats = [p for per in pairs for p, _ in per]
+        reps = [r for per in pairs for _, r in per]
+        m = len(pats)
+        doc_ptrs = (C.c_char_p * max(n, 1))(*docs)
+        doc_lens = (c_size_t * max(n, 1))(*map(len, docs))
+        counts = (C.c_int * max(n, 1))(*map(len, pairs))
+        pat_ptrs = (C.c_char_p * max(m, 1))(*pats)
+        pat_lens = (C.c_int * max(m, 1))(*map(len, pats))
+        rep_ptrs = (C.c_char_p * max(m, 1))(*reps)
+        rep_lens = (C.c_int * max(m, 1))(*map(len, reps))
         base = c_uint8_p()
         offs = (c_size_t * max(n, 1))()
         lens = (c_size_t * max(n, 1))()
@@ -325,6 +331,10 @@ class BatchEngine:
         self.last_stats = {"device_ms": stats[0], "wall_ms": stats[1], "algo_bytes": stats[2],
                            "rounds": int(stats[3])}
         addr = C.cast(base, c_void_p).value or 0
+        if not copy:
+            total = max((offs[i] + lens[i] for i in range(n)), default=0)
+            whole = memoryview((C.c_uint8 * total).from_address(addr)).cast("B") if total else memoryview(b"")
+            return [whole[offs[i]:offs[i] + lens[i]].toreadonly() for i in range(n)]
         return [C.string_at(addr + offs[i], lens[i]) if lens[i] else b"" for i in range(n)]
 
     def close(self) -> None:
