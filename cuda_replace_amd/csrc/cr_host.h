@@ -1,0 +1,32 @@
+// cr_host.h -- host-side helpers shared by the runtime files.
+#pragma once
+#include <stddef.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+namespace cr {
+
+// memcpy split over up to 8 threads for large copies (pageable host memory into
+// pinned staging runs at a few GB/s per thread).
+inline void parallel_memcpy(void *dst, const void *src, size_t n)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = n < (16u << 20) ? 1 : (int)std::min<unsigned>(hw ? hw : 1, 8);
+    if (nt <= 1) {
+        if (n)
+            memcpy(dst, src, n);
+        return;
+    }
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nt; ++t) {
+        const size_t a = n * t / nt, b = n * (t + 1) / nt;
+        ts.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, b - a); });
+    }
+    for (auto &t : ts)
+        t.join();
+}
+
+} // namespace cr

@@ -12,6 +12,7 @@
 // 64-bit lengths and no per-call device allocation of pattern buffers.
 #include "cr_internal.h"
 #include "cr_pattern.h"
+#include "cr_host.h"
 #include "../../include/cuda_replace.h"
 
 #include <limits.h>
@@ -21,6 +22,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
+#include <algorithm>
+#include <chrono>
 
 using namespace cr;
 
@@ -64,6 +67,17 @@ struct Session {
     LaunchStats *d_stats = nullptr;
     int64_t *d_query = nullptr;
     uint32_t epoch = 0;
+
+    // streaming pipeline (cr_stream_replace): two chunk slots in and out on the
+    // device, pinned staging for pageable input, upload / download streams
+    uint8_t *d_sin[2] = {nullptr, nullptr};
+    size_t cap_sin[2] = {0, 0};
+    uint8_t *d_sout[2] = {nullptr, nullptr};
+    size_t cap_sout[2] = {0, 0};
+    uint8_t *h_sin[2] = {nullptr, nullptr};
+    size_t cap_hsin[2] = {0, 0};
+    hipStream_t cs_in = nullptr, cs_out = nullptr;
+    hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
 
     // record of the most recent apply, for prefix queries
     bool q_valid = false;
@@ -591,6 +605,227 @@ int session_upload(Session *s, const uint8_t *src, size_t len)
     return 0;
 }
 
+
+// ---- streaming pipeline ---------------------------------------------------------
+// Pinned host output that grows; in-flight downloads into it finish first.
+struct PinnedOut {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    size_t len = 0;
+};
+
+hipError_t pinned_reserve(PinnedOut &o, size_t need, hipStream_t pending)
+{
+    if (o.p && o.cap >= need)
+        return hipSuccess;
+    hipError_t e = hipStreamSynchronize(pending);
+    if (e != hipSuccess)
+        return e;
+    const size_t nc = round_up(std::max(need, o.cap + o.cap / 2), 1u << 20);
+    void *np = nullptr;
+    e = hipHostMalloc(&np, nc, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return e;
+    if (o.p) {
+        parallel_memcpy(np, o.p, o.len);
+        (void)hipHostFree(o.p);
+    }
+    o.p = (uint8_t *)np;
+    o.cap = nc;
+    return hipSuccess;
+}
+
+hipError_t ensure_stream_pipeline(Session *s)
+{
+    if (s->cs_in)
+        return hipSuccess;
+    hipError_t e = hipStreamCreateWithFlags(&s->cs_in, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&s->cs_out, hipStreamNonBlocking);
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        e = hipEventCreateWithFlags(&s->ev_in[b], hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&s->ev_out[b], hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipEventRecord(s->ev_in[b], s->cs_in);
+        if (e == hipSuccess)
+            e = hipEventRecord(s->ev_out[b], s->cs_out);
+    }
+    return e;
+}
+
+// One pair over X (n bytes) in chunks of C source bytes: chunk k covers source
+// [kC, (k+1)C) plus an (L-1)-byte overlap, its seed is the last kept start before
+// kC, and its output runs from the end of a match straddling kC (dropped bytes)
+// to the end of a match straddling (k+1)C (cuda_replace_wrapper.py:305-366, with
+// the cut moved to the end of a straddling match so no byte is lost).  Chunk
+// k+1 is staged and uploaded while chunk k computes; chunk k's output is
+// downloaded into Y while chunk k+1 computes.
+int stream_pair(Session *s, const uint8_t *X, size_t n, bool x_pinned, const uint8_t *pat, int L,
+                const uint8_t *rep, int r, size_t C, PinnedOut &Y, int64_t *chunks)
+{
+    const int64_t diff = (int64_t)r - L;
+    Y.len = 0;
+    CK(pinned_reserve(Y, (diff <= 0 ? n : n + n / 16) + (size_t)r + 64, s->cs_out));
+    if (n == 0) {
+        if (L == 0 && r > 0) {
+            memcpy(Y.p, rep, (size_t)r);
+            Y.len = (size_t)r;
+        }
+        return 0;
+    }
+    const size_t H = L > 0 ? (size_t)L - 1 : 0;
+    const size_t K = (n + C - 1) / C;
+    constexpr int64_t kNone = INT64_MIN;
+    int64_t last_global = kNone;   // last kept start of the chunks so far
+    size_t cut_prev = 0;           // source position where the previous chunk's output ended
+
+    auto upload = [&](size_t k) -> int {
+        const int b = (int)(k & 1);
+        const size_t start = k * C;
+        const size_t len = std::min(n, start + C + H) - start;
+        // the slot's previous upload (chunk k-2) was consumed by a finished pass
+        CK(grow((void **)&s->d_sin[b], &s->cap_sin[b], len + 16, false, s->stream));
+        const uint8_t *src = X + start;
+        if (!x_pinned) {
+            CK(hipEventSynchronize(s->ev_in[b]));   // staging slot b is free again
+            CK(grow_host(&s->h_sin[b], &s->cap_hsin[b], len + 16));
+            parallel_memcpy(s->h_sin[b], src, len);
+            src = s->h_sin[b];
+        }
+        CK(hipMemcpyAsync(s->d_sin[b], src, len, hipMemcpyHostToDevice, s->cs_in));
+        CK(hipEventRecord(s->ev_in[b], s->cs_in));
+        return 0;
+    };
+
+    int rc = upload(0);
+    if (rc != 0)
+        return rc;
+    for (size_t k = 0; k < K; ++k) {
+        const int b = (int)(k & 1);
+        const size_t start = k * C;
+        const size_t len = std::min(n, start + C + H) - start;
+        const bool last_chunk = k + 1 == K;
+        if (!last_chunk && (rc = upload(k + 1)) != 0)
+            return rc;
+        CK(hipStreamWaitEvent(s->stream, s->ev_in[b], 0));    // chunk k is on the device
+        CK(hipStreamWaitEvent(s->stream, s->ev_out[b], 0));   // slot b's last download is done
+        int64_t floor_pos = 0;
+        if (last_global != kNone) {
+            const int64_t f = last_global + L - (int64_t)start;
+            floor_pos = f > 0 ? f : 0;
+        }
+        PassResult res;
+        int mode = 0;
+        ReplaceParams qp{};
+        auto dst_for = [&](int64_t need, uint8_t **dst) -> int {
+            hipError_t e = hipEventSynchronize(s->ev_out[b]);
+            if (e == hipSuccess)
+                e = grow((void **)&s->d_sout[b], &s->cap_sout[b], (size_t)(need > 0 ? need : 1) + 16, false,
+                         s->stream);
+            if (e != hipSuccess)
+                return (int)e;
+            *dst = s->d_sout[b];
+            return 0;
+        };
+        rc = run_pass(s, s->d_sin[b], (int64_t)len, pat, L, rep, r, floor_pos, -1, false, dst_for, &res, nullptr,
+                      &mode, &qp);
+        if (rc != 0)
+            return rc;
+        // output slice of this chunk
+        const size_t drop = cut_prev > start ? cut_prev - start : 0;
+        size_t end;
+        size_t cut;
+        if (L == 0) {
+            // rep b0 rep b1 ... rep: the trailing rep belongs to the next chunk
+            end = last_chunk ? (size_t)res.out_len : len * ((size_t)r + 1);
+            cut = start + len;
+        } else if (last_chunk) {
+            end = (size_t)res.out_len;
+            cut = n;
+        } else {
+            CK(launch_query(qp, mode, (int64_t)C, s->d_query, s->stream));
+            CK(hipMemcpyAsync(s->h_misc + 128, s->d_query, 16, hipMemcpyDeviceToHost, s->stream));
+            CK(hipStreamSynchronize(s->stream));
+            const int64_t kept_before = ((const int64_t *)(s->h_misc + 128))[0];
+            const int64_t last_rel = ((const int64_t *)(s->h_misc + 128))[1];
+            if (last_rel >= 0)
+                last_global = (int64_t)start + last_rel;
+            // the latest kept match may cover past the chunk end (also one from an
+            // earlier chunk when L exceeds the chunk size)
+            cut = start + C;
+            if (last_global != kNone)
+                cut = std::max(cut, (size_t)(last_global + L));
+            // no kept start lies in [C, cut - start): it would overlap the straddling one
+            end = (size_t)((int64_t)(cut - start) + kept_before * diff);
+        }
+        if (end > drop) {
+            CK(pinned_reserve(Y, Y.len + (end - drop) + 64, s->cs_out));
+            CK(hipMemcpyAsync(Y.p + Y.len, s->d_sout[b] + drop, end - drop, hipMemcpyDeviceToHost, s->cs_out));
+            Y.len += end - drop;
+        }
+        CK(hipEventRecord(s->ev_out[b], s->cs_out));
+        cut_prev = cut;
+        if (chunks)
+            ++*chunks;
+    }
+    CK(hipStreamSynchronize(s->cs_out));
+    return 0;
+}
+
+// Pairs in order, each streamed over the previous pair's output.
+int session_stream(Session *s, const uint8_t *src, size_t n, const uint8_t *const *pats, const int *pat_lens,
+                   const uint8_t *const *reps, const int *rep_lens, int count, size_t chunk, uint8_t **out,
+                   size_t *out_len, double *stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    CK(ensure_stream_pipeline(s));
+    if (chunk == 0)
+        chunk = 256u << 20;
+    PinnedOut cur;
+    const uint8_t *X = src;
+    size_t xn = n;
+    bool x_pinned = false;
+    double bytes_in = 0, bytes_out = 0;
+    int64_t chunks = 0;
+    for (int i = 0; i < count; ++i) {
+        PinnedOut Y;
+        int rc = stream_pair(s, X, xn, x_pinned, pats[i], pat_lens[i], reps ? reps[i] : nullptr,
+                             reps ? rep_lens[i] : 0, chunk, Y, &chunks);
+        if (rc != 0) {
+            if (Y.p)
+                (void)hipHostFree(Y.p);
+            if (cur.p)
+                (void)hipHostFree(cur.p);
+            return rc;
+        }
+        bytes_in += (double)xn;
+        bytes_out += (double)Y.len;
+        if (cur.p)
+            (void)hipHostFree(cur.p);
+        cur = Y;
+        X = cur.p;
+        xn = cur.len;
+        x_pinned = true;
+    }
+    if (count == 0) {
+        CK(pinned_reserve(cur, n + 64, s->cs_out));
+        parallel_memcpy(cur.p, src, n);
+        cur.len = n;
+    }
+    *out = cur.len ? cur.p : nullptr;
+    *out_len = cur.len;
+    if (!cur.len && cur.p)
+        (void)hipHostFree(cur.p);
+    if (stats) {
+        stats[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats[1] = bytes_in;
+        stats[2] = bytes_out;
+        stats[3] = (double)chunks;
+    }
+    return 0;
+}
+
 void destroy(Session *s)
 {
     if (!s)
@@ -598,10 +833,23 @@ void destroy(Session *s)
     DeviceGuard g(s->device);
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
-    void *dev[] = {s->d_buf, s->d_tmp, s->d_gsum, s->d_chunks, s->d_pre, s->d_info, s->d_bitmap, s->d_patrep, s->d_presence, s->d_misc};
+    for (hipStream_t cs : {s->cs_in, s->cs_out})
+        if (cs)
+            (void)hipStreamSynchronize(cs);
+    void *dev[] = {s->d_buf, s->d_tmp, s->d_gsum, s->d_chunks, s->d_pre, s->d_info, s->d_bitmap, s->d_patrep,
+                   s->d_presence, s->d_misc, s->d_sin[0], s->d_sin[1], s->d_sout[0], s->d_sout[1]};
     for (void *p : dev)
         if (p)
             (void)hipFree(p);
+    for (uint8_t *p : {s->h_sin[0], s->h_sin[1]})
+        if (p)
+            (void)hipHostFree(p);
+    for (hipEvent_t e : {s->ev_in[0], s->ev_in[1], s->ev_out[0], s->ev_out[1]})
+        if (e)
+            (void)hipEventDestroy(e);
+    for (hipStream_t cs : {s->cs_in, s->cs_out})
+        if (cs)
+            (void)hipStreamDestroy(cs);
     if (s->h_patrep)
         (void)hipHostFree(s->h_patrep);
     if (s->h_misc)
@@ -652,7 +900,30 @@ int open_session(Session **out)
 // =============================================================================
 extern "C" {
 
-const char *cuda_replace_version(void) { return "cuda_replace_amd 1.0 gfx950 single-pass"; }
+int cr_stream_replace(void *h, const uint8_t *src, size_t n, const uint8_t *const *pats, const int *pat_lens,
+                      const uint8_t *const *reps, const int *rep_lens, int count, size_t chunk_bytes,
+                      uint8_t **out_host, size_t *out_len, double *stats)
+{
+    if (!h || !out_host || !out_len || (n > 0 && !src) || count < 0 || (count > 0 && (!pats || !pat_lens)))
+        return -1;
+    for (int i = 0; i < count; ++i) {
+        if (pat_lens[i] < 0 || (pat_lens[i] > 0 && !pats[i]))
+            return -1;
+        if (reps && (!rep_lens || rep_lens[i] < 0 || (rep_lens[i] > 0 && !reps[i])))
+            return -1;
+    }
+    Session *s = (Session *)h;
+    std::lock_guard<std::mutex> lock(s->mtx);
+    if (!s->alive)
+        return -3;
+    DeviceGuard g(s->device);
+    *out_host = nullptr;
+    *out_len = 0;
+    s->q_valid = false;   // the pipeline's passes are not the session's applies
+    return session_stream(s, src, n, pats, pat_lens, reps, rep_lens, count, chunk_bytes, out_host, out_len, stats);
+}
+
+const char *cuda_replace_version(void) { return "cuda_replace_amd 1.0 gfx950 count-spine-emit"; }
 
 int cuda_replace_open(void **ph, const uint8_t *src, size_t len)
 {

@@ -42,7 +42,7 @@ REQUIRED_SYMBOLS = (
     "cuda_replace_query_prefix", "cuda_replace_apply_batch", "cuda_replace_result",
     "cuda_replace_close", "cuda_replace_version",
     "cr_session_length", "cr_query_prefix64", "cr_replace_device", "cr_pattern_mode",
-    "cr_batch_create", "cr_batch_run", "cr_batch_destroy",
+    "cr_batch_create", "cr_batch_run", "cr_batch_destroy", "cr_stream_replace",
 )
 
 
@@ -114,6 +114,10 @@ class CudaReplaceLib:
         lib.cr_batch_run.restype = C.c_int
         lib.cr_batch_destroy.argtypes = [c_void_p]
         lib.cr_batch_destroy.restype = None
+        lib.cr_stream_replace.argtypes = [c_void_p, C.c_char_p, c_size_t, c_char_pp, c_int_p, c_char_pp, c_int_p,
+                                          C.c_int, c_size_t, C.POINTER(c_uint8_p), c_size_t_p,
+                                          C.POINTER(C.c_double)]
+        lib.cr_stream_replace.restype = C.c_int
 
     def version(self) -> str:
         return self.lib.cuda_replace_version().decode()
@@ -237,54 +241,37 @@ def replace_unified(lib: CudaReplaceLib, src: bytes, pat: bytes, rep: bytes) -> 
 
 
 def gpu_replace_streaming(lib: CudaReplaceLib, src: bytes, pairs: Iterable[Tuple[bytes, bytes]],
-                          chunk_bytes: int = 256 * 1024 * 1024) -> bytes:
-    """Memory-bounded chunked replace with an exact carry between chunks
-    (protocol of py:305-366): each chunk is taken with an L-1 byte overlap,
-    applied with the previous chunk's last kept start as seed, and cut after the
-    kept matches that start before the chunk end."""
+                          chunk_bytes: int = 256 * 1024 * 1024, stats: Optional[dict] = None) -> bytes:
+    """Memory-bounded chunked replace (py:305-366) through the library's pipelined
+    streaming (``cr_stream_replace``): per pair, chunks of ``chunk_bytes`` with an
+    L-1 byte overlap, the seed carried from chunk to chunk, each chunk's output cut
+    after the kept match that straddles its end; the next chunk uploads while the
+    current one computes.  The result equals ``bytes.replace`` of the whole input
+    (empty patterns included).  ``stats``, when given, receives wall ms, bytes in
+    and out summed over pairs, and the number of chunks."""
     if chunk_bytes <= 0:
         chunk_bytes = 1
-    out = src
+    pairs = list(pairs)
+    k = len(pairs)
+    pats = [p for p, _ in pairs]
+    reps = [r for _, r in pairs]
+    out_ptr = c_uint8_p()
+    out_len = c_size_t()
+    st = (C.c_double * 4)()
     with Session(lib, b"") as sess:
-        for pat, rep in pairs:
-            L = len(pat)
-            n = len(out)
-            pieces: List[bytes] = []
-            if L == 0:
-                # bytes.replace(b"", rep): rep before every byte and once at the end
-                pos = 0
-                while True:
-                    end = min(pos + chunk_bytes, n)
-                    sess.reset(out[pos:end])
-                    sess.apply(pat, rep)
-                    chunk_out = sess.result()
-                    if end < n:
-                        pieces.append(chunk_out[:len(chunk_out) - len(rep)])
-                        pos = end
-                    else:
-                        pieces.append(chunk_out)
-                        break
-                out = b"".join(pieces)
-                continue
-            diff = len(rep) - L
-            pos = 0
-            prev_last: Optional[int] = None
-            while pos < n:
-                base_end = min(pos + chunk_bytes, n)
-                take_end = min(base_end + L - 1, n)
-                sess.reset(out[pos:take_end])
-                seed = -L if prev_last is None else prev_last - pos
-                sess.apply_seeded(pat, rep, seed)
-                kept_before, last_before = sess.query_prefix(base_end - pos)
-                cut = base_end - pos
-                if last_before >= 0:
-                    cut = max(cut, last_before + L)
-                gpu_out = sess.result()
-                pieces.append(gpu_out[:cut + kept_before * diff])
-                if last_before >= 0:
-                    prev_last = pos + last_before
-                pos += cut
-            out = b"".join(pieces)
+        rc = lib.lib.cr_stream_replace(sess.h, src, len(src), (C.c_char_p * max(k, 1))(*pats),
+                                       (C.c_int * max(k, 1))(*map(len, pats)), (C.c_char_p * max(k, 1))(*reps),
+                                       (C.c_int * max(k, 1))(*map(len, reps)), k, chunk_bytes,
+                                       C.byref(out_ptr), C.byref(out_len), st)
+        if rc != 0:
+            raise RuntimeError(f"cr_stream_replace failed rc={rc}")
+        try:
+            out = C.string_at(out_ptr, out_len.value) if out_len.value else b""
+        finally:
+            if out_ptr:
+                lib.lib.cuda_free_host(out_ptr)
+    if stats is not None:
+        stats.update({"wall_ms": st[0], "bytes_in": st[1], "bytes_out": st[2], "chunks": int(st[3])})
     return out
 
 

@@ -126,6 +126,22 @@ CR_API int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d
  * tables (self-overlapping, length <= 16), 3 bitmap + serial greedy, 0 empty. */
 CR_API int cr_pattern_mode(const uint8_t *pat, int pat_len);
 
+/* Streams a host buffer through the GPU: for each pair in order, the current data
+ * passes in chunks of chunk_bytes source bytes (0 -> 256 MiB) with an (L-1)-byte
+ * overlap, the seed carried from chunk to chunk and each chunk's output cut at the
+ * end of a kept match straddling its end, so the result equals bytes.replace of the
+ * whole input (replaces the protocol of cuda_replace_wrapper.py:305-366, which cuts
+ * at the chunk end and loses bytes there).  The next chunk is staged in pinned
+ * memory and uploaded while the current one computes, and each chunk's output is
+ * downloaded while the next computes.  *out_host is a pinned buffer freed with
+ * cuda_free_host (NULL when *out_len == 0).  stats (optional, 4 doubles): wall ms,
+ * bytes streamed in, bytes streamed out (summed over pairs), chunks processed.
+ * Uses the session's stream and device; leaves the session's buffer untouched. */
+CR_API int cr_stream_replace(void *h, const uint8_t *src, size_t n, const uint8_t *const *pats,
+                             const int *pat_lens, const uint8_t *const *reps, const int *rep_lens,
+                             int count, size_t chunk_bytes, uint8_t **out_host, size_t *out_len,
+                             double *stats);
+
 /* ---- batched documents (no counterpart in the reference; its callers loop over
  * cuda_replace_unified per document and pair, cuda_replace.cu:2579-2601) ----- */
 
