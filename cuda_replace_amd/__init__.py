@@ -5,7 +5,8 @@ built in-tree by ``python -m cuda_replace_amd.build``).  This package exposes it
 through the reference wrapper's API.
 """
 from .wrapper import (DEFAULT_LIB, REQUIRED_SYMBOLS, BatchEngine, CudaReplaceLib,  # noqa: F401
-                      Session, gpu_replace_streaming, replace_unified)
+                      Server, Session, Ticket, gpu_replace_streaming, replace_unified)
 
-__all__ = ["CudaReplaceLib", "Session", "BatchEngine", "replace_unified", "gpu_replace_streaming",
+__all__ = ["CudaReplaceLib", "Session", "BatchEngine", "Server", "Ticket", "replace_unified",
+           "gpu_replace_streaming",
            "DEFAULT_LIB", "REQUIRED_SYMBOLS"]

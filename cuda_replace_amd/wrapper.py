@@ -43,6 +43,7 @@ REQUIRED_SYMBOLS = (
     "cuda_replace_close", "cuda_replace_version",
     "cr_session_length", "cr_query_prefix64", "cr_replace_device", "cr_pattern_mode",
     "cr_batch_create", "cr_batch_run", "cr_batch_destroy", "cr_stream_replace",
+    "cr_server_create", "cr_server_submit", "cr_server_wait", "cr_server_release", "cr_server_destroy",
 )
 
 
@@ -118,6 +119,17 @@ class CudaReplaceLib:
                                           C.c_int, c_size_t, C.POINTER(c_uint8_p), c_size_t_p,
                                           C.POINTER(C.c_double)]
         lib.cr_stream_replace.restype = C.c_int
+        lib.cr_server_create.argtypes = [C.POINTER(c_void_p), c_size_t, C.c_int, c_size_t]
+        lib.cr_server_create.restype = C.c_int
+        lib.cr_server_submit.argtypes = [c_void_p, C.c_char_p, c_size_t, C.c_int, c_char_pp, c_int_p, c_char_pp,
+                                         c_int_p, C.POINTER(c_void_p)]
+        lib.cr_server_submit.restype = C.c_int
+        lib.cr_server_wait.argtypes = [c_void_p, C.POINTER(c_uint8_p), c_size_t_p, C.POINTER(C.c_double)]
+        lib.cr_server_wait.restype = C.c_int
+        lib.cr_server_release.argtypes = [c_void_p]
+        lib.cr_server_release.restype = None
+        lib.cr_server_destroy.argtypes = [c_void_p]
+        lib.cr_server_destroy.restype = None
 
     def version(self) -> str:
         return self.lib.cuda_replace_version().decode()
@@ -340,3 +352,69 @@ class BatchEngine:
             self.close()
         except Exception:
             pass
+
+
+class Ticket:
+    """One request queued on a ``Server``; ``result()`` blocks until it is done."""
+
+    def __init__(self, server: "Server", handle: c_void_p, keep: tuple):
+        self._server = server
+        self._h = handle
+        self._keep = keep          # the request's bytes stay alive until the result is read
+        self._out: Optional[bytes] = None
+        self.times: Optional[dict] = None
+
+    def result(self) -> bytes:
+        if self._out is None:
+            out = c_uint8_p()
+            n = c_size_t()
+            t = (C.c_double * 3)()
+            rc = self._server.lib.lib.cr_server_wait(self._h, C.byref(out), C.byref(n), t)
+            try:
+                if rc != 0:
+                    raise RuntimeError(f"cr_server_wait failed rc={rc}")
+                self._out = C.string_at(out, n.value) if n.value else b""
+                self.times = {"queue_ms": t[0], "latency_ms": t[1], "batch_docs": int(t[2])}
+            finally:
+                self._server.lib.lib.cr_server_release(self._h)
+                self._h = c_void_p()
+                self._keep = ()
+        return self._out
+
+
+class Server:
+    """Continuous-batching request server (``cr_server_*``): requests are admitted
+    in arrival order under an HBM budget and run in batches by ``workers`` batch
+    engines, each on its own stream.  Each result equals applying the request's
+    pairs in order with ``bytes.replace``."""
+
+    def __init__(self, lib: CudaReplaceLib, hbm_budget: int = 0, workers: int = 2, max_batch_bytes: int = 0):
+        self.lib = lib
+        self.h = c_void_p()
+        rc = lib.lib.cr_server_create(C.byref(self.h), hbm_budget, workers, max_batch_bytes)
+        if rc != 0:
+            raise RuntimeError(f"cr_server_create failed rc={rc}")
+
+    def submit(self, doc: bytes, pairs: Sequence[Tuple[bytes, bytes]]) -> Ticket:
+        k = len(pairs)
+        pats = [p for p, _ in pairs]
+        reps = [r for _, r in pairs]
+        pat_arr = (C.c_char_p * max(k, 1))(*pats)
+        rep_arr = (C.c_char_p * max(k, 1))(*reps)
+        t = c_void_p()
+        rc = self.lib.lib.cr_server_submit(self.h, doc, len(doc), k, pat_arr, (C.c_int * max(k, 1))(*map(len, pats)),
+                                           rep_arr, (C.c_int * max(k, 1))(*map(len, reps)), C.byref(t))
+        if rc != 0:
+            raise RuntimeError(f"cr_server_submit failed rc={rc}")
+        return Ticket(self, t, (doc, pats, reps))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.lib.cr_server_destroy(self.h)
+            self.h = c_void_p()
+
+    def __enter__(self) -> "Server":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()

@@ -168,6 +168,31 @@ CR_API int cr_batch_run(void *b, int num_docs, const uint8_t *const *docs, const
 /* Destroys a batch engine; NULL is ignored. */
 CR_API void cr_batch_destroy(void *b);
 
+/* ---- continuous-batching server (no counterpart in the reference) ---------- */
+
+/* Starts a server on the current device with `workers` batch engines (0 -> 2),
+ * an HBM budget for admitted requests (0 -> 200 GB) and a per-batch cap (0 -> 4 GiB).
+ * Each request holds 3 x its input + 1 MiB of budget while its batch runs. */
+CR_API int cr_server_create(void **ps, size_t hbm_budget_bytes, int workers, size_t max_batch_bytes);
+
+/* Queues one request: doc and its ordered (pattern, replacement) pairs, with
+ * bytes.replace semantics per pair.  The caller keeps doc and the pattern and
+ * replacement bytes alive until cr_server_wait returns for *ticket. */
+CR_API int cr_server_submit(void *srv, const uint8_t *doc, size_t len, int npairs,
+                            const uint8_t *const *pats, const int *pat_lens,
+                            const uint8_t *const *reps, const int *rep_lens, void **ticket);
+
+/* Blocks until the request is done; *out / *out_len get its result (owned by the
+ * ticket).  times (optional, 3 doubles): ms from submit to its batch's start, ms
+ * from submit to completion, documents in its batch. */
+CR_API int cr_server_wait(void *ticket, const uint8_t **out, size_t *out_len, double *times);
+
+/* Frees a ticket and its result (waits for completion first). */
+CR_API void cr_server_release(void *ticket);
+
+/* Completes every queued request, then stops the workers and frees the server. */
+CR_API void cr_server_destroy(void *srv);
+
 #ifdef __cplusplus
 }
 #endif
