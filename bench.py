@@ -20,6 +20,14 @@ batched  = the uneven-documents workload (8192 documents, log-normal sizes, 1-16
            against the oracle; beside it the per-call loop of the reference's callers
            (one cuda_replace_unified per document and pair) timed on every 32nd
            document.  Reported as an extra field of the same line.
+serving  = open-loop Poisson arrivals (1000 requests/s for --serve-seconds) of
+           requests sliced from the text (log-normal sizes, median 1 MiB, sigma 2,
+           4 KiB - 256 MiB; 1-32 pairs each) into the continuous-batching server
+           (4 workers, 200 GB budget): p50 / p99 latency from scheduled arrival to
+           completion, aggregate input GB/s, a sample checked against the oracle.
+streaming = the text repeated to --stream-gib GiB, streamed through the pipelined
+           chunked path in 256 MiB chunks (fragment -> 12 bytes), host to host,
+           checked against bytes.replace of the 1 GiB piece.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--size-mib 1024]
 """
@@ -138,6 +146,105 @@ def batch_bench(lib, ro, num_docs: int) -> dict:
     }
 
 
+def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed: int = 7, workers: int = 4) -> dict:
+    """Open-loop Poisson arrivals into the server (see module docstring)."""
+    import numpy as np
+    from cuda_replace_amd import Server, workloads
+    rng = np.random.default_rng(seed)
+    nreq = max(1, int(seconds * rate))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    sizes = np.clip(np.rint(rng.lognormal(np.log(1 << 20), 2.0, nreq)), 4096, min(256 << 20, len(text)))
+    sizes = sizes.astype(np.int64)
+    offs = [int(rng.integers(0, len(text) - int(sz) + 1)) for sz in sizes]
+    pool = workloads.pair_pool(text[: 4 << 20], 256, seed=8)
+    pairs = [[pool[int(k)] for k in rng.integers(0, len(pool), int(c))] for c in rng.integers(1, 33, nreq)]
+    check = set(int(i) for i in rng.choice(np.flatnonzero(sizes <= (8 << 20)), size=min(40, nreq), replace=False))
+    lag = np.zeros(nreq)
+    tickets = []
+    with Server(lib, hbm_budget=200 << 30, workers=workers) as srv:
+        srv.submit_at(text, 0, 1 << 20, pairs[0]).result()      # warm the engines
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            target = t0 + float(arrivals[i])
+            now = time.perf_counter()
+            if target > now:
+                time.sleep(target - now)
+            lag[i] = max(0.0, time.perf_counter() - target)
+            tickets.append(srv.submit_at(text, offs[i], int(sizes[i]), pairs[i]))
+        lat = np.zeros(nreq)
+        exact = True
+        batch_docs = []
+        for i, t in enumerate(tickets):
+            view = t.wait()
+            if i in check:
+                o = offs[i]
+                exact &= bytes(view) == ro.apply_batch(text[o:o + int(sizes[i])], pairs[i])
+            lat[i] = lag[i] * 1e3 + t.times["latency_ms"]
+            batch_docs.append(t.times["batch_docs"])
+            t.release()
+        wall = time.perf_counter() - t0
+    log(f"[bench] serving: {nreq} requests in {wall:.2f}s, p50 {np.percentile(lat, 50):.2f} ms, "
+        f"p99 {np.percentile(lat, 99):.2f} ms, bit_exact(sample)={exact}")
+    return {
+        "workload": f"open-loop Poisson {rate:.0f} req/s for {seconds:.1f}s; log-normal sizes median 1 MiB, "
+                    f"sigma 2, 4 KiB-256 MiB (slices of the text); 1-32 pairs each; {workers} workers, 200 GB budget",
+        "requests": nreq,
+        "input_bytes": int(sizes.sum()),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_batch_docs": round(float(np.mean(batch_docs)), 1),
+        "max_submit_lag_ms": round(float(lag.max() * 1e3), 3),
+        "offered_GBps": round(float(sizes.sum()) / float(arrivals[-1]) / 1e9, 3),
+        "achieved_GBps": round(float(sizes.sum()) / wall / 1e9, 3),
+        "bit_exact": bool(exact),
+        "checked_sample": len(check),
+    }
+
+
+def stream_bench(lib, text: bytes, frag: bytes, rep: bytes, piece: bytes, gib: int) -> dict:
+    """The text repeated to `gib` GiB through the pipelined streaming path; the
+    result is checked in the library's pinned output buffer (no copy into Python)."""
+    import ctypes as C
+    from cuda_replace_amd import Session
+    L = len(frag)
+    seam = text[-(L - 1):] + text[:L - 1]
+    assert seam.find(frag) == -1, "a match would span two copies of the text"
+    reps = max(1, (gib << 30) // len(text))
+    big = text * reps
+    out_ptr = C.POINTER(C.c_uint8)()
+    out_len = C.c_size_t()
+    st = (C.c_double * 4)()
+    with Session(lib, b"") as sess:
+        t = time.perf_counter()
+        rc = lib.lib.cr_stream_replace(sess.h, big, len(big), (C.c_char_p * 1)(frag), (C.c_int * 1)(L),
+                                       (C.c_char_p * 1)(rep), (C.c_int * 1)(len(rep)), 1, 256 << 20,
+                                       C.byref(out_ptr), C.byref(out_len), st)
+        wall = time.perf_counter() - t
+        if rc != 0:
+            raise RuntimeError(f"cr_stream_replace rc={rc}")
+        n_out = out_len.value
+        addr = C.cast(out_ptr, C.c_void_p).value
+        exact = n_out == len(piece) * reps
+        if exact:
+            mv = memoryview((C.c_uint8 * n_out).from_address(addr)).cast("B")
+            exact = all(mv[i * len(piece):(i + 1) * len(piece)] == piece for i in range(reps))
+            del mv
+        lib.lib.cuda_free_host(out_ptr)
+    log(f"[bench] streaming: {len(big) / 2**30:.0f} GiB in {wall:.2f}s ({len(big) / wall / 1e9:.2f} GB/s), "
+        f"{int(st[3])} chunks, bit_exact={exact}")
+    del big
+    return {
+        "workload": f"{reps} x the 1 GiB text, fragment -> 12 B, 256 MiB chunks, pinned double buffers, "
+                    f"upload / compute / download on separate streams, pageable host input to pinned host output",
+        "input_bytes": len(text) * reps,
+        "output_bytes": n_out,
+        "wall_s": round(wall, 3),
+        "input_GBps": round(len(text) * reps / wall / 1e9, 3),
+        "chunks": int(st[3]),
+        "bit_exact": bool(exact),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +254,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-documents measurement")
     ap.add_argument("--batch-docs", type=int, default=8192)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming measurement")
+    ap.add_argument("--stream-gib", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the replace path runs on a single MI355X "
@@ -260,6 +372,9 @@ def main() -> int:
         log(f"[bench] {name}: out_len={ol} bit_exact={ok}")
     lib.lib.cuda_replace_close(h)
     batched = None if args.no_batch else batch_bench(lib, ro, args.batch_docs)
+    serving = None if args.no_serve else serve_bench(lib, ro, text, args.serve_seconds, workers=args.serve_workers)
+    streaming = None if args.no_stream else stream_bench(lib, text, frag, rep_e, expected["text_expand"],
+                                                         args.stream_gib)
 
     total_in = 3 * n * args.steps
     value = total_in / wall / 1e9
@@ -319,10 +434,13 @@ def main() -> int:
                          "cpu": platform.processor() or platform.machine()},
         "device_copy_GBps": round(copy_gbs, 1),
         "batched": batched,
+        "serving": serving,
+        "streaming": streaming,
         "detail": detail,
     }
     print(json.dumps(line), flush=True)
-    return 0 if exact and (batched is None or batched["bit_exact"]) else 1
+    ok = exact and all(x is None or x["bit_exact"] for x in (batched, serving, streaming))
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

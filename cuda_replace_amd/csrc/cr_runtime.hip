@@ -666,7 +666,7 @@ int stream_pair(Session *s, const uint8_t *X, size_t n, bool x_pinned, const uin
 {
     const int64_t diff = (int64_t)r - L;
     Y.len = 0;
-    CK(pinned_reserve(Y, (diff <= 0 ? n : n + n / 16) + (size_t)r + 64, s->cs_out));
+    CK(pinned_reserve(Y, (diff <= 0 ? n : n + n / 8) + (size_t)r + 64, s->cs_out));
     if (n == 0) {
         if (L == 0 && r > 0) {
             memcpy(Y.p, rep, (size_t)r);

@@ -2,7 +2,8 @@
 //
 // Requests (one document and its ordered pair list) enter a FIFO.  Each worker
 // thread owns a batch engine (its own HIP stream, resident arenas and pinned
-// staging).  When a worker is free it admits queued requests in arrival order while
+// staging); four workers by default, so one batch's host packing and copies
+// overlap other batches' transfers and kernels.  When a worker is free it admits queued requests in arrival order while
 // the server's HBM budget and the batch size cap allow, runs them as one batch, and
 // completes each request with its own copy of the result.  Requests that arrive
 // while batches are on the GPU wait only until a worker frees up, and with two or
@@ -12,6 +13,7 @@
 //
 // No counterpart in the reference (a library without a serving layer).
 #include "../../include/cuda_replace.h"
+#include "cr_host.h"
 
 #include <hip/hip_runtime.h>
 
@@ -136,7 +138,7 @@ void worker_loop(Server *S, int w)
             if (rc == 0 && olens[i] > 0) {
                 t->out = (uint8_t *)malloc(olens[i]);
                 if (t->out) {
-                    memcpy(t->out, base + offs[i], olens[i]);
+                    cr::parallel_memcpy(t->out, base + offs[i], olens[i]);
                     t->out_len = olens[i];
                 } else {
                     trc = -2;
@@ -186,7 +188,7 @@ CR_API int cr_server_create(void **ps, size_t hbm_budget_bytes, int workers, siz
     }
     S->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)200 << 30;
     S->max_batch = max_batch_bytes ? max_batch_bytes : (size_t)4 << 30;
-    const int nw = workers > 0 ? workers : 2;
+    const int nw = workers > 0 ? workers : 4;
     for (int w = 0; w < nw; ++w) {
         void *eng = nullptr;
         int rc = cr_batch_create(&eng);

@@ -47,6 +47,20 @@ REQUIRED_SYMBOLS = (
 )
 
 
+_bytes_from = C.pythonapi.PyBytes_FromStringAndSize
+_bytes_from.restype = C.py_object
+_bytes_from.argtypes = [c_void_p, C.c_ssize_t]
+
+
+def _bytes_at(ptr, n: int) -> bytes:
+    """Copies n bytes at ptr into a new bytes object; any size (ctypes.string_at
+    takes a C int length and cuts results of 2 GiB and more)."""
+    if not n:
+        return b""
+    addr = ptr if isinstance(ptr, int) else C.cast(ptr, c_void_p).value
+    return _bytes_from(addr, n)
+
+
 def _u8(b: bytes):
     """Pointer to the bytes object's own buffer (no copy; NULs allowed)."""
     return C.cast(C.c_char_p(b), c_uint8_p)
@@ -121,7 +135,7 @@ class CudaReplaceLib:
         lib.cr_stream_replace.restype = C.c_int
         lib.cr_server_create.argtypes = [C.POINTER(c_void_p), c_size_t, C.c_int, c_size_t]
         lib.cr_server_create.restype = C.c_int
-        lib.cr_server_submit.argtypes = [c_void_p, C.c_char_p, c_size_t, C.c_int, c_char_pp, c_int_p, c_char_pp,
+        lib.cr_server_submit.argtypes = [c_void_p, c_void_p, c_size_t, C.c_int, c_char_pp, c_int_p, c_char_pp,
                                          c_int_p, C.POINTER(c_void_p)]
         lib.cr_server_submit.restype = C.c_int
         lib.cr_server_wait.argtypes = [c_void_p, C.POINTER(c_uint8_p), c_size_t_p, C.POINTER(C.c_double)]
@@ -146,7 +160,7 @@ class CudaReplaceLib:
         if rc != 0:
             raise RuntimeError(f"cuda_replace_unified failed rc={rc}")
         try:
-            return C.string_at(out_ptr, out_len.value) if out_len.value else b""
+            return _bytes_at(out_ptr, out_len.value)
         finally:
             self.lib.cuda_free_host(out_ptr)
 
@@ -230,7 +244,7 @@ class Session:
         if rc != 0:
             raise RuntimeError(f"cuda_replace_result failed rc={rc}")
         try:
-            return C.string_at(out_ptr, out_len.value) if out_len.value else b""
+            return _bytes_at(out_ptr, out_len.value)
         finally:
             self.lib.lib.cuda_free_host(out_ptr)
 
@@ -278,7 +292,7 @@ def gpu_replace_streaming(lib: CudaReplaceLib, src: bytes, pairs: Iterable[Tuple
         if rc != 0:
             raise RuntimeError(f"cr_stream_replace failed rc={rc}")
         try:
-            out = C.string_at(out_ptr, out_len.value) if out_len.value else b""
+            out = _bytes_at(out_ptr, out_len.value)
         finally:
             if out_ptr:
                 lib.lib.cuda_free_host(out_ptr)
@@ -334,7 +348,7 @@ class BatchEngine:
             total = max((offs[i] + lens[i] for i in range(n)), default=0)
             whole = memoryview((C.c_uint8 * total).from_address(addr)).cast("B") if total else memoryview(b"")
             return [whole[offs[i]:offs[i] + lens[i]].toreadonly() for i in range(n)]
-        return [C.string_at(addr + offs[i], lens[i]) if lens[i] else b"" for i in range(n)]
+        return [_bytes_at(addr + offs[i], lens[i]) for i in range(n)]
 
     def close(self) -> None:
         if self.h:
@@ -364,6 +378,31 @@ class Ticket:
         self._out: Optional[bytes] = None
         self.times: Optional[dict] = None
 
+    def wait(self) -> memoryview:
+        """Blocks until done and returns a read-only view of the result, valid
+        until ``release()``; ``times`` is filled in."""
+        if self._out is not None:
+            return memoryview(self._out)
+        out = c_uint8_p()
+        n = c_size_t()
+        t = (C.c_double * 3)()
+        rc = self._server.lib.lib.cr_server_wait(self._h, C.byref(out), C.byref(n), t)
+        if rc != 0:
+            self.release()
+            raise RuntimeError(f"cr_server_wait failed rc={rc}")
+        self.times = {"queue_ms": t[0], "latency_ms": t[1], "batch_docs": int(t[2])}
+        addr = C.cast(out, c_void_p).value
+        if not n.value:
+            return memoryview(b"")
+        return memoryview((C.c_uint8 * n.value).from_address(addr)).cast("B").toreadonly()
+
+    def release(self) -> None:
+        """Frees the request and its result (views from ``wait()`` become invalid)."""
+        if self._h:
+            self._server.lib.lib.cr_server_release(self._h)
+            self._h = c_void_p()
+            self._keep = ()
+
     def result(self) -> bytes:
         if self._out is None:
             out = c_uint8_p()
@@ -373,7 +412,7 @@ class Ticket:
             try:
                 if rc != 0:
                     raise RuntimeError(f"cr_server_wait failed rc={rc}")
-                self._out = C.string_at(out, n.value) if n.value else b""
+                self._out = _bytes_at(out, n.value)
                 self.times = {"queue_ms": t[0], "latency_ms": t[1], "batch_docs": int(t[2])}
             finally:
                 self._server.lib.lib.cr_server_release(self._h)
@@ -388,7 +427,7 @@ class Server:
     engines, each on its own stream.  Each result equals applying the request's
     pairs in order with ``bytes.replace``."""
 
-    def __init__(self, lib: CudaReplaceLib, hbm_budget: int = 0, workers: int = 2, max_batch_bytes: int = 0):
+    def __init__(self, lib: CudaReplaceLib, hbm_budget: int = 0, workers: int = 4, max_batch_bytes: int = 0):
         self.lib = lib
         self.h = c_void_p()
         rc = lib.lib.cr_server_create(C.byref(self.h), hbm_budget, workers, max_batch_bytes)
@@ -396,17 +435,24 @@ class Server:
             raise RuntimeError(f"cr_server_create failed rc={rc}")
 
     def submit(self, doc: bytes, pairs: Sequence[Tuple[bytes, bytes]]) -> Ticket:
+        return self.submit_at(doc, 0, len(doc), pairs)
+
+    def submit_at(self, buf: bytes, offset: int, length: int, pairs: Sequence[Tuple[bytes, bytes]]) -> Ticket:
+        """Queues ``buf[offset:offset + length]`` without copying it."""
+        if offset < 0 or length < 0 or offset + length > len(buf):
+            raise ValueError("slice outside the buffer")
         k = len(pairs)
         pats = [p for p, _ in pairs]
         reps = [r for _, r in pairs]
         pat_arr = (C.c_char_p * max(k, 1))(*pats)
         rep_arr = (C.c_char_p * max(k, 1))(*reps)
         t = c_void_p()
-        rc = self.lib.lib.cr_server_submit(self.h, doc, len(doc), k, pat_arr, (C.c_int * max(k, 1))(*map(len, pats)),
+        addr = C.cast(C.c_char_p(buf), c_void_p).value + offset
+        rc = self.lib.lib.cr_server_submit(self.h, addr, length, k, pat_arr, (C.c_int * max(k, 1))(*map(len, pats)),
                                            rep_arr, (C.c_int * max(k, 1))(*map(len, reps)), C.byref(t))
         if rc != 0:
             raise RuntimeError(f"cr_server_submit failed rc={rc}")
-        return Ticket(self, t, (doc, pats, reps))
+        return Ticket(self, t, (buf, pats, reps))
 
     def close(self) -> None:
         if self.h:

@@ -170,7 +170,7 @@ CR_API void cr_batch_destroy(void *b);
 
 /* ---- continuous-batching server (no counterpart in the reference) ---------- */
 
-/* Starts a server on the current device with `workers` batch engines (0 -> 2),
+/* Starts a server on the current device with `workers` batch engines (0 -> 4),
  * an HBM budget for admitted requests (0 -> 200 GB) and a per-batch cap (0 -> 4 GiB).
  * Each request holds 3 x its input + 1 MiB of budget while its batch runs. */
 CR_API int cr_server_create(void **ps, size_t hbm_budget_bytes, int workers, size_t max_batch_bytes);
