@@ -60,6 +60,21 @@ __device__ __forceinline__ uint32_t greedy32(uint32_t m, int c, int L, int *last
     return kept;
 }
 
+// The same greedy for a self-overlapping 2-byte pattern (its two bytes are equal,
+// so occurrences overlap by one): in every run of consecutive candidates the kept
+// starts are the candidates at an even distance from the run's first one.  Adding
+// the odd-positioned run starts to x carries through exactly the runs that start
+// at odd positions.
+__device__ __forceinline__ uint32_t greedy32_pair(uint32_t m, int c, int *last)
+{
+    const uint32_t x = c >= 32 ? 0u : (m & (0xffffffffu << c));
+    const uint32_t starts = x & ~(x << 1);
+    const uint32_t odd_runs = ((x + (starts & 0xAAAAAAAAu)) ^ x) & x;
+    const uint32_t kept = ((x & ~odd_runs) & 0x55555555u) | (odd_runs & 0xAAAAAAAAu);
+    *last = kept ? 31 - __builtin_clz(kept) : kNegInf;
+    return kept;
+}
+
 // ---- carry tables (self-overlapping patterns) -------------------------------
 // entry = carry_out << 24 | kept_count ; carry < 16, count <= 8192 inside a tile
 template <int LT>
@@ -387,7 +402,8 @@ __device__ __forceinline__ Table<LT> segment_table(uint32_t m, int L)
 #pragma unroll
     for (int c = 0; c < LT; ++c) {
         int lst;
-        const uint32_t km = greedy32(m, c, L, &lst);
+        // two carry states: the pattern is 2 bytes long (L <= LT) and self-overlapping
+        const uint32_t km = LT == 2 ? greedy32_pair(m, c, &lst) : greedy32(m, c, L, &lst);
         int co = lst + L - 32;
         co = co < 0 ? 0 : co;
         t.e[c] = ((uint32_t)co << 24) | (uint32_t)__builtin_popcount(km);
@@ -487,7 +503,7 @@ constexpr int kSpineWaves = kSpineThreads / kWave;
 
 template <int W>
 struct SpineShape {
-    static constexpr int V = W <= 2 ? 16 : (W <= 8 ? 8 : 4);   // groups per lane and pass
+    static constexpr int V = W == 1 ? 16 : (16 / W > 1 ? 16 / W : 1);   // groups per lane and pass
 };
 
 template <int MODE, int LT>
@@ -558,16 +574,18 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
         } else {
             // carry tables, two sweeps over j: first the wave's total table, then,
             // once the state entering the wave's block is known, each group's
-            // exclusive state from an ordered wave scan (summaries reloaded from L2)
-            auto load_table = [&](int64_t gi) {
-                Table<LT> t = tidentity<LT>();
+            // exclusive state from an ordered wave scan; the summaries are loaded once
+            Table<LT> tv[V];
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int64_t gi = wb + (int64_t)j * kWave + lane;
+                tv[j] = tidentity<LT>();
                 if (gi < groups) {
 #pragma unroll
                     for (int c = 0; c < LT; ++c)
-                        t.e[c] = p.gsum[(first_group + gi) * LT + c];
+                        tv[j].e[c] = p.gsum[(first_group + gi) * LT + c];
                 }
-                return widen<LT>(t);
-            };
+            }
             auto wave_scan = [&](Table64<LT> inc) {
 #pragma unroll
                 for (int d = 1; d < kWave; d <<= 1) {
@@ -581,11 +599,9 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
                 return inc;
             };
             Table64<LT> run = tidentity64<LT>();
-#pragma unroll 1
-            for (int j = 0; j < V; ++j) {
-                const Table64<LT> t = load_table(wb + (int64_t)j * kWave + lane);
-                run = tcompose64<LT>(run, wave_broadcast0<LT>(wave_compose<LT>(t, lane)));
-            }
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+                run = tcompose64<LT>(run, wave_broadcast0<LT>(wave_compose<LT>(widen<LT>(tv[j]), lane)));
             if (lane == 0) {
 #pragma unroll
                 for (int c = 0; c < LT; ++c)
@@ -605,10 +621,10 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
             const uint64_t ew = tsel64<LT>(wpre, (uint32_t)carry);
             int64_t kw = k + (int64_t)(ew & kCountMask);   // state entering the j-block
             uint32_t cw = (uint32_t)(ew >> kCarryShift);
-#pragma unroll 1
+#pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int64_t gi = wb + (int64_t)j * kWave + lane;
-                const Table64<LT> inc = wave_scan(load_table(gi));
+                const Table64<LT> inc = wave_scan(widen<LT>(tv[j]));
                 uint64_t e = (uint64_t)cw << kCarryShift;   // lane 0: nothing before it
 #pragma unroll
                 for (int c = 0; c < LT; ++c) {
@@ -834,7 +850,7 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
             k_loc = (int)(me & 0xffffffu);
             c_next = last_tile ? 0 : (int)(tsel<LT>(agg, (uint32_t)c_t) >> 24);
             int lst;
-            kept = greedy32(m, c_i, L, &lst);
+            kept = LT == 2 ? greedy32_pair(m, c_i, &lst) : greedy32(m, c_i, L, &lst);
             const int lastq = kept ? seg + lst : kNegInf;
             int ks = __builtin_popcount(kept), lq = lastq;
 #pragma unroll
