@@ -55,7 +55,41 @@ enum Mode : int32_t {
     kModeBorderFree = 1,   // pattern cannot overlap itself: every candidate >= F is kept
     kModeTable = 2,        // self-overlapping, L <= 16: per-carry-state tables
     kModeBitmap = 3,       // general: candidate bitmap, serial greedy, scatter
+    kModeFused = 5,        // several pairs in one pass (see FusedTable)
 };
+
+// ---- fused multi-pattern pass ----------------------------------------------------
+// Several (pattern, replacement) pairs applied in one pass.  The host only builds a
+// table when that equals applying the pairs one after another: the patterns are
+// border-free, distinct, none a substring of another and no proper suffix of one a
+// prefix of another (so no two occurrences of any of them overlap and every
+// occurrence is kept), every replacement is non-empty and made of bytes that occur in
+// no pattern (so no replacement forms or joins a later occurrence), and all patterns
+// share their first byte (the candidate filter).
+constexpr int kMaxFusedPats = 256;
+constexpr int kMaxFusedPat = 64;       // longest pattern or replacement in a fused pass
+constexpr int kFusedPool = 16384;      // pattern + replacement bytes
+constexpr int kFusedHashBits = 10;
+constexpr int kFusedSlots = 1 << kFusedHashBits;
+
+struct FusedTable {
+    int32_t count;
+    int32_t key_len;                   // pattern bytes after the first one that are hashed (1..4)
+    int32_t min_len;
+    int32_t max_len;
+    uint16_t slot[kFusedSlots];        // pattern index + 1, 0 = empty; linear probing
+    uint16_t pat_len[kMaxFusedPats];
+    uint16_t rep_len[kMaxFusedPats];
+    uint16_t pat_off[kMaxFusedPats];   // into pool
+    uint16_t rep_off[kMaxFusedPats];
+    uint8_t pool[kFusedPool];
+};
+
+// Hash of the key bytes after the shared first byte (multiplicative, top bits).
+__host__ __device__ inline uint32_t fused_hash(uint32_t key)
+{
+    return (key * 2654435761u) >> (32 - kFusedHashBits);
+}
 
 // stats words written by a pass
 struct LaunchStats {
@@ -125,6 +159,9 @@ struct BatchArgs {
 hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s);
 hipError_t launch_batch_spine(const BatchArgs &a, int mode, int table_size, int64_t docs, hipStream_t s);
 hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s);
+
+hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, int64_t first_group,
+                        int64_t groups, hipStream_t s);   // which: 0 count, 1 spine, 2 emit
 
 // ---- launchers (cr_kernels.hip) --------------------------------------------
 // One chunk of groups [first_group, first_group + groups): count, spine, emit.

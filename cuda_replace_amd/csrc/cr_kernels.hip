@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(kSpineThreads)
 spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk, int last_chunk)
 {
     constexpr int W = MODE == kModeTable ? LT : 1;   // words per group summary
-    constexpr int V = SpineShape<W>::V;
+    constexpr int V = MODE == kModeFused ? 8 : SpineShape<W>::V;
     constexpr int64_t kBlock = (int64_t)kWave * V;   // groups per wave and pass
     constexpr int64_t kPass = kBlock * kSpineWaves;
     __shared__ uint64_t s_wt[kSpineWaves][LT];
@@ -529,7 +529,49 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
 
     for (int64_t base = 0; base < groups; base += kPass) {
         const int64_t wb = base + (int64_t)wave * kBlock;   // wave's first group (chunk-relative)
-        if constexpr (MODE != kModeTable) {
+        if constexpr (MODE == kModeFused) {
+            // signed length changes (replacement - pattern bytes) per group
+            int64_t v[V], ex[V];
+            int64_t run = 0;
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int64_t gi = wb + (int64_t)j * kWave + lane;
+                v[j] = gi < groups ? (int64_t)(int32_t)p.gsum[first_group + gi] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                int64_t inc = v[j];
+#pragma unroll
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const int64_t u = __shfl_up(inc, d, kWave);
+                    if (lane >= d)
+                        inc += u;
+                }
+                ex[j] = run + inc - v[j];
+                run += __shfl(inc, kWave - 1, kWave);
+            }
+            if (lane == 0)
+                s_wsum[wave] = run;
+            __syncthreads();
+            int64_t before = 0, total = 0;
+            for (int w = 0; w < kSpineWaves; ++w) {
+                if (w < wave)
+                    before += s_wsum[w];
+                total += s_wsum[w];
+            }
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int64_t gi = wb + (int64_t)j * kWave + lane;
+                if (gi < groups) {
+                    ChunkState cs;
+                    cs.k_base = k + before + ex[j];
+                    cs.carry = 0;
+                    cs.pad = 0;
+                    p.chunks_pre[first_group + gi] = cs;
+                }
+            }
+            k += total;
+        } else if constexpr (MODE != kModeTable) {
             // kept counts: per j an inclusive wave scan, offset by the running sum
             uint32_t v[V], ex[V];
             uint32_t run = 0;
@@ -655,8 +697,8 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
         cs.carry = carry;
         cs.pad = 0;
         p.chunks[chunk + 1] = cs;
-        if (last_chunk)
-            p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | (uint64_t)k;
+        if (last_chunk)   // 48-bit two's complement payload (a fused pass's total may be negative)
+            p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | ((uint64_t)k & ((1ull << kEpochShift) - 1));
     }
 }
 
@@ -1069,6 +1111,261 @@ batch_spine_kernel(BatchArgs a)
     }
     if (lane == 0)
         a.doc_kept[doc] = k;
+}
+
+// ---- fused multi-pattern pass ------------------------------------------------
+// Index of the pattern occurring at sp (LDS; avail source bytes from there), or -1.
+// The caller has matched the shared first byte; the key is the next key_len bytes.
+__device__ __forceinline__ int fused_probe(const FusedTable *t, const uint8_t *sp, int64_t avail)
+{
+    if (avail < t->min_len)
+        return -1;
+    const int kl = t->key_len;
+    uint32_t key = 0;
+    for (int b = 0; b < kl; ++b)
+        key |= (uint32_t)sp[1 + b] << (8 * b);
+    uint32_t h = fused_hash(key);
+    for (int probe = 0; probe < kFusedSlots; ++probe) {
+        const int sl = t->slot[h];
+        if (sl == 0)
+            return -1;
+        const int k = sl - 1;
+        const int L = t->pat_len[k];
+        if (L <= avail) {
+            const uint8_t *pp = t->pool + t->pat_off[k];
+            bool ok = true;
+            for (int i = 1; i < L; ++i) {
+                if (sp[i] != pp[i]) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok)
+                return k;
+        }
+        h = (h + 1) & (kFusedSlots - 1);
+    }
+    return -1;
+}
+
+// Group summary: the signed length change sum(rep_len - pat_len) over every
+// occurrence starting in the group (all occurrences are kept).
+__device__ __forceinline__ void fused_count_group(const ReplaceParams &p, const FusedTable *t, int64_t g,
+                                                  int32_t *summary)
+{
+    __shared__ __align__(16) uint8_t s_in[kInBytes];
+    __shared__ int32_t s_red[kWaves];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int64_t t0 = g * kGroupTiles;
+    if (t0 >= p.num_tiles)
+        return;
+    load_group(p, t0, tid, false, s_in);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int seg = tid * kBytesPerThread;
+    int32_t delta = 0;
+    for (int i = 0; i < kGroupTiles; ++i) {
+        const int64_t ti = t0 + i;
+        if (ti >= p.num_tiles)
+            break;
+        const int64_t gseg = ti * kTile + seg;
+        const uint8_t *sp = s_in + kHalo + i * kTile + seg;
+        uint32_t m = cand_mask(p, s_in + kHalo + i * kTile, s_in, seg, gseg);
+        while (m) {
+            const int q = __builtin_ctz(m);
+            m &= m - 1;
+            const int k = fused_probe(t, sp + q, p.n - (gseg + q));
+            if (k >= 0)
+                delta += (int32_t)t->rep_len[k] - (int32_t)t->pat_len[k];
+        }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1)
+        delta += __shfl_xor(delta, d, kWave);
+    if (lane == 0)
+        s_red[wave] = delta;
+    __syncthreads();
+    if (tid == 0)
+        summary[0] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+// Copies len bytes from HBM to LDS (any alignment): 16-byte loads, exact tail.
+__device__ __forceinline__ void global_to_lds(uint8_t *dst, const uint8_t *src, int len)
+{
+    for (; len >= 16; len -= 16, src += 16, dst += 16) {
+        uint4 v;
+        __builtin_memcpy(&v, src, 16);
+        lds_store16(dst, v);
+    }
+    for (int i = 0; i < len; ++i)
+        dst[i] = src[i];
+}
+
+// Emits one group of a fused pass: every occurrence of every pattern is replaced.
+// pre->k_base is the signed length change of everything before the group.
+__device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const FusedTable *t, int64_t g,
+                                                 const ChunkState *pre)
+{
+    __shared__ __align__(16) uint8_t s_in[kInBytes];
+    __shared__ __align__(16) uint8_t s_out[kStage];
+    __shared__ int s_wsum[kWaves];
+    __shared__ int s_wmax[kWaves];
+    __shared__ int s_halo_end;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int64_t n = p.n;
+    const int64_t nt = p.num_tiles;
+    const int64_t t0 = g * kGroupTiles;
+    if (t0 >= nt)
+        return;
+    if (tid == 0)
+        s_halo_end = 0;
+    load_group(p, t0, tid, true, s_in);
+    int64_t k_run = pre->k_base;   // length change before the current tile
+    int c_run = 0;                 // leading bytes of the current tile covered by an earlier match
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int seg = tid * kBytesPerThread;
+    const uint8_t first = (uint8_t)p.pat4;
+    for (int i = 0; i < kGroupTiles; ++i) {
+        const int64_t ti = t0 + i;
+        if (ti >= nt)
+            break;
+        const int64_t start = ti * kTile;
+        const int valid = (int)((n - start) < kTile ? (n - start) : kTile);
+        const bool last_tile = ti == nt - 1;
+        const int seg_valid = valid - seg;
+        const int jend = seg_valid < 32 ? (seg_valid > 0 ? seg_valid : 0) : 32;
+        const int in_off = kHalo + i * kTile;
+        const uint8_t *sp = s_in + in_off + seg;
+        // my occurrences: kept mask, summed length change, end of the last one
+        uint32_t m = cand_mask(p, s_in + in_off, s_in, seg, start + seg);
+        uint32_t kept = 0;
+        int d = 0;
+        int e = kNegInf;
+        while (m) {
+            const int q = __builtin_ctz(m);
+            m &= m - 1;
+            const int k = fused_probe(t, sp + q, n - (start + seg + q));
+            if (k >= 0) {
+                kept |= 1u << q;
+                d += (int)t->rep_len[k] - (int)t->pat_len[k];
+                e = seg + q + (int)t->pat_len[k];
+            }
+        }
+        if (i == 0) {
+            // an occurrence starting in the max_len-1 bytes left of the group covers
+            // the group's first bytes
+            if (ti > 0 && tid < t->max_len - 1) {
+                const int q = -1 - tid;
+                if (s_in[in_off + q] == first) {
+                    const int k = fused_probe(t, s_in + in_off + q, n - (start + q));
+                    if (k >= 0 && q + (int)t->pat_len[k] > 0)
+                        atomicMax(&s_halo_end, q + (int)t->pat_len[k]);
+                }
+            }
+            __syncthreads();
+            c_run = s_halo_end;
+        }
+        const int c_t = c_run;
+        // in-tile scans: length change (sum) and coverage end (max)
+        int ds = d, le = e;
+#pragma unroll
+        for (int dd = 1; dd < kWave; dd <<= 1) {
+            const int ud = __shfl_up(ds, dd, kWave);
+            const int ue = __shfl_up(le, dd, kWave);
+            if (lane >= dd) {
+                ds += ud;
+                le = ue > le ? ue : le;
+            }
+        }
+        int le_ex = __shfl_up(le, 1, kWave);
+        if (lane == 0)
+            le_ex = kNegInf;
+        if (lane == kWave - 1) {
+            s_wsum[wave] = ds;
+            s_wmax[wave] = le;
+        }
+        __syncthreads();
+        int wsum = 0, wmax = c_t, tile_d = 0, tile_end = c_t;
+        for (int w = 0; w < kWaves; ++w) {
+            if (w < wave) {
+                wsum += s_wsum[w];
+                wmax = s_wmax[w] > wmax ? s_wmax[w] : wmax;
+            }
+            tile_d += s_wsum[w];
+            tile_end = s_wmax[w] > tile_end ? s_wmax[w] : tile_end;
+        }
+        const int d_loc = wsum + ds - d;
+        const int prev_end = le_ex > wmax ? le_ex : wmax;
+        int c_i = prev_end - seg;
+        if (c_i < 0)
+            c_i = 0;
+        int c_next = tile_end - kTile;
+        if (c_next < 0 || last_tile)
+            c_next = 0;
+        if (p.write_output) {
+            const int64_t out_len = (int64_t)valid + c_next - c_t + tile_d;
+            uint8_t *const out_first = p.dst + start + c_t + k_run;
+            const bool staged = out_len + 32 <= kStage;
+            int64_t o = (seg + c_i - c_t) + d_loc;
+            int j = c_i;
+            uint32_t M = j >= 32 ? 0u : (kept & (0xffffffffu << j));
+            while (j < jend) {
+                const int q = M ? __builtin_ctz(M) : jend;
+                if (q > j) {
+                    if (staged)
+                        lds_move(s_out + o, sp + j, q - j);
+                    else
+                        store_from_lds(out_first + o, s_in, in_off + seg + j, q - j);
+                    o += q - j;
+                }
+                if (q >= jend)
+                    break;
+                const int k = fused_probe(t, sp + q, n - (start + seg + q));
+                const int r = t->rep_len[k];
+                const uint8_t *rp = t->pool + t->rep_off[k];
+                if (staged)
+                    global_to_lds(s_out + o, rp, r);
+                else
+                    store_from_global(out_first + o, rp, r);
+                o += r;
+                j = q + (int)t->pat_len[k];
+                M = j >= 32 ? 0u : (M & (0xffffffffu << j));
+            }
+            if (staged) {
+                __syncthreads();
+                const int len = (int)out_len;
+                for (int c = tid * 16; c < len; c += kThreads * 16) {
+                    const uint4 v = *reinterpret_cast<const uint4 *>(s_out + c);
+                    if (c + 16 <= len)
+                        __builtin_memcpy(out_first + c, &v, 16);
+                    else
+                        store_tail(out_first + c, v, len - c);
+                }
+            }
+        }
+        k_run += tile_d;
+        c_run = c_next;
+        __syncthreads();   // s_out, s_wsum and s_wmax are reused by the next tile
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+fused_count_kernel(ReplaceParams p, const FusedTable *t, int64_t first_group)
+{
+    const int64_t g = first_group + blockIdx.x;
+    fused_count_group(p, t, g, reinterpret_cast<int32_t *>(p.gsum) + g);
+}
+
+__global__ void __launch_bounds__(kThreads)
+fused_emit_kernel(ReplaceParams p, const FusedTable *t, int64_t first_group)
+{
+    const int64_t g = first_group + blockIdx.x;
+    fused_emit_group(p, t, g, p.chunks_pre + g);
 }
 
 // ---- bitmap mode: candidate bitmap -------------------------------------------
@@ -1546,6 +1843,21 @@ hipError_t launch_batch_spine(const BatchArgs &a, int mode, int table_size, int6
 hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s)
 {
     return batch_dispatch(a, mode, table_size, 2, groups, s);
+}
+
+hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, int64_t first_group, int64_t groups,
+                        hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    if (which == 0)
+        hipLaunchKernelGGL(fused_count_kernel, dim3((unsigned)groups), dim3(kThreads), 0, s, p, t, first_group);
+    else if (which == 1)
+        hipLaunchKernelGGL((spine_kernel<kModeFused, 1>), dim3(1), dim3(kSpineThreads), 0, s, p, first_group, groups,
+                           (int64_t)0, 1);
+    else
+        hipLaunchKernelGGL(fused_emit_kernel, dim3((unsigned)groups), dim3(kThreads), 0, s, p, t, first_group);
+    return hipGetLastError();
 }
 
 } // namespace cr

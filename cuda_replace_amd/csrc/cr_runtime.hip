@@ -79,6 +79,16 @@ struct Session {
     hipStream_t cs_in = nullptr, cs_out = nullptr;
     hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
 
+    // fused apply_batch: device table; the pairs and the batch's source (kept in
+    // d_tmp) let a later prefix query replay the pairs one by one
+    FusedTable *d_fused = nullptr;
+    FusedTable *h_fused = nullptr;   // pinned staging
+    bool fused_pending = false;
+    std::vector<std::vector<uint8_t>> fused_pats, fused_reps;
+    size_t fused_src_len = 0;
+    uint8_t *d_q[2] = {nullptr, nullptr};
+    size_t cap_q[2] = {0, 0};
+
     // record of the most recent apply, for prefix queries
     bool q_valid = false;
     int q_mode = 0;
@@ -548,6 +558,7 @@ int session_apply(Session *s, const uint8_t *pat, int L, const uint8_t *rep, int
         return 0;
     };
     s->q_valid = false;
+    s->fused_pending = false;
     int rc = run_pass(s, s->d_buf, (int64_t)s->len, pat, L, rep, r, floor_pos, -1, out_last != nullptr, dst_for,
                       &res, nullptr, &mode, &qp);
     if (rc != 0)
@@ -569,10 +580,17 @@ int session_apply(Session *s, const uint8_t *pat, int L, const uint8_t *rep, int
     return 0;
 }
 
+int replay_fused(Session *s);
+
 int session_query(Session *s, size_t limit, int64_t *count, int64_t *last)
 {
     *count = 0;
     *last = -1;
+    if (s->fused_pending) {
+        const int rc = replay_fused(s);
+        if (rc != 0)
+            return rc;
+    }
     if (!s->q_valid)
         return 0;
     const int64_t n = s->q_params.n;
@@ -602,6 +620,219 @@ int session_upload(Session *s, const uint8_t *src, size_t len)
     CK(hipStreamSynchronize(s->stream));
     s->len = len;
     s->q_valid = false;
+    s->fused_pending = false;
+    return 0;
+}
+
+
+// ---- fused multi-pattern apply_batch -----------------------------------------------
+// Builds the fused table when one pass over the source equals applying the pairs in
+// order: 2..kMaxFusedPats pairs; patterns of 2..kMaxFusedPat bytes sharing their first
+// byte, border-free, distinct, none a substring of another, no proper suffix of one
+// equal to a prefix of another (so no two occurrences of any of them overlap and every
+// occurrence is kept); replacements of 1..kMaxFusedPat bytes that contain no byte of
+// any pattern (so no replacement forms or completes an occurrence of a later pattern,
+// and none is deleted to join bytes around it).  Returns false otherwise.
+bool build_fused(const uint8_t *const *pats, const int *pat_lens, const uint8_t *const *reps, const int *rep_lens,
+                 int count, FusedTable *t)
+{
+    if (count < 2 || count > kMaxFusedPats || !pats || !pat_lens || !reps || !rep_lens)
+        return false;
+    bool in_pat[256] = {false};
+    int min_len = INT_MAX, max_len = 0;
+    size_t pool = 0;
+    for (int i = 0; i < count; ++i) {
+        const int L = pat_lens[i], r = rep_lens[i];
+        if (L < 2 || L > kMaxFusedPat || r < 1 || r > kMaxFusedPat || !pats[i] || !reps[i])
+            return false;
+        if (pats[i][0] != pats[0][0] || !border_free(pats[i], L))
+            return false;
+        for (int k = 0; k < L; ++k)
+            in_pat[pats[i][k]] = true;
+        min_len = std::min(min_len, L);
+        max_len = std::max(max_len, L);
+        pool += (size_t)L + (size_t)r;
+    }
+    if (pool > (size_t)kFusedPool)
+        return false;
+    for (int i = 0; i < count; ++i)
+        for (int k = 0; k < rep_lens[i]; ++k)
+            if (in_pat[reps[i][k]])
+                return false;
+    const uint8_t c0 = pats[0][0];
+    for (int a = 0; a < count; ++a) {
+        const uint8_t *A = pats[a];
+        const int La = pat_lens[a];
+        for (int b = 0; b < count; ++b) {
+            if (a == b)
+                continue;
+            const uint8_t *B = pats[b];
+            const int Lb = pat_lens[b];
+            // A inside B (every occurrence of A starts with c0) ...
+            for (int x = 0; x + La <= Lb; ++x)
+                if (B[x] == c0 && memcmp(B + x, A, (size_t)La) == 0)
+                    return false;
+            // ... or a proper suffix of A equal to a prefix of B
+            for (int x = 1; x < La; ++x) {
+                const int l = La - x;
+                if (l < Lb && A[x] == c0 && memcmp(A + x, B, (size_t)l) == 0)
+                    return false;
+            }
+        }
+    }
+    memset(t, 0, sizeof(FusedTable));
+    t->count = count;
+    t->key_len = std::min(4, min_len - 1);
+    t->min_len = min_len;
+    t->max_len = max_len;
+    uint32_t off = 0;
+    for (int i = 0; i < count; ++i) {
+        t->pat_len[i] = (uint16_t)pat_lens[i];
+        t->rep_len[i] = (uint16_t)rep_lens[i];
+        t->pat_off[i] = (uint16_t)off;
+        memcpy(t->pool + off, pats[i], (size_t)pat_lens[i]);
+        off += (uint32_t)pat_lens[i];
+        t->rep_off[i] = (uint16_t)off;
+        memcpy(t->pool + off, reps[i], (size_t)rep_lens[i]);
+        off += (uint32_t)rep_lens[i];
+        uint32_t key = 0;
+        for (int b = 0; b < t->key_len; ++b)
+            key |= (uint32_t)pats[i][1 + b] << (8 * b);
+        uint32_t h = fused_hash(key);
+        while (t->slot[h] != 0)
+            h = (h + 1) & (kFusedSlots - 1);
+        t->slot[h] = (uint16_t)(i + 1);
+    }
+    return true;
+}
+
+// One fused pass over the session buffer with the table staged in s->h_fused.
+int session_apply_fused(Session *s)
+{
+    CK(ensure_misc(s));
+    if (!s->d_fused)
+        CK(hipMalloc((void **)&s->d_fused, sizeof(FusedTable)));
+    CK(hipMemcpyAsync(s->d_fused, s->h_fused, sizeof(FusedTable), hipMemcpyHostToDevice, s->stream));
+    const int64_t n = (int64_t)s->len;
+    int64_t out = n;
+    if (n > 0) {
+        const int64_t num_tiles = (n + kTile - 1) / kTile;
+        const int64_t num_groups = (num_tiles + kGroupTiles - 1) / kGroupTiles;
+        CK(ensure_scratch(s, num_tiles, kModeBorderFree, 0));
+        CK(next_epoch(s));
+        ReplaceParams p{};
+        p.src = s->d_buf;
+        p.n = n;
+        p.floor = 0;
+        p.out_len = -1;
+        p.L = 1;
+        p.pat4 = s->h_fused->pool[0];
+        p.mask4 = 0xffu;
+        p.gsum = s->d_gsum;
+        p.chunks = s->d_chunks;
+        p.chunks_pre = s->d_pre;
+        p.stats = s->d_stats;
+        p.epoch = s->epoch;
+        p.write_output = 1;
+        p.num_tiles = num_tiles;
+        p.num_groups = num_groups;
+        CK(launch_fused(p, s->d_fused, 0, 0, num_groups, s->stream));
+        CK(launch_fused(p, s->d_fused, 1, 0, num_groups, s->stream));
+        CK(hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        const uint64_t kt = ((const LaunchStats *)(s->h_misc + 64))->k_total;
+        if ((uint32_t)(kt >> kEpochShift) != s->epoch)
+            return (int)hipErrorUnknown;
+        const int64_t delta = (int64_t)(kt << (64 - kEpochShift)) >> (64 - kEpochShift);   // sign-extend
+        out = n + delta;
+        CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(out > 0 ? out : 1), false, s->stream));
+        p.dst = s->d_tmp;
+        CK(launch_fused(p, s->d_fused, 2, 0, num_groups, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+    } else {
+        CK(grow((void **)&s->d_tmp, &s->cap_tmp, 1, false, s->stream));
+    }
+    std::swap(s->d_buf, s->d_tmp);
+    std::swap(s->cap_buf, s->cap_tmp);
+    s->fused_src_len = (size_t)n;
+    s->len = (size_t)out;
+    return 0;
+}
+
+// Pairs in order; fused into one pass when build_fused allows (and allow_fuse).
+int session_apply_batch(Session *s, const uint8_t *const *pats, const int *pat_lens, const uint8_t *const *reps,
+                        const int *rep_lens, int count, bool allow_fuse, int *fused_out)
+{
+    if (fused_out)
+        *fused_out = 0;
+    if (allow_fuse && count >= 2) {
+        if (!s->h_fused)
+            CK(hipHostMalloc((void **)&s->h_fused, sizeof(FusedTable), hipHostMallocDefault));
+        CK(hipStreamSynchronize(s->stream));   // the staging may still feed an earlier upload
+        if (build_fused(pats, pat_lens, reps, rep_lens, count, s->h_fused)) {
+            s->q_valid = false;
+            s->fused_pending = false;
+            int rc = session_apply_fused(s);
+            if (rc != 0)
+                return rc;
+            s->fused_pats.assign(count, {});
+            s->fused_reps.assign(count, {});
+            for (int i = 0; i < count; ++i) {
+                s->fused_pats[i].assign(pats[i], pats[i] + pat_lens[i]);
+                s->fused_reps[i].assign(reps[i], reps[i] + rep_lens[i]);
+            }
+            s->fused_pending = true;
+            if (fused_out)
+                *fused_out = 1;
+            return 0;
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        const uint8_t *r = (reps && rep_lens) ? reps[i] : nullptr;
+        const int rl = (reps && rep_lens) ? rep_lens[i] : 0;
+        int rc = session_apply(s, pats[i], pat_lens[i], r, rl, 0, nullptr, nullptr);
+        if (rc != 0)
+            return rc;
+    }
+    return 0;
+}
+
+// A prefix query after a fused batch refers to the batch's last pair, applied to the
+// output of the ones before it: replay the pairs one by one from the batch's source
+// (still in d_tmp) into two scratch buffers, keeping the last pass's record.
+int replay_fused(Session *s)
+{
+    const int count = (int)s->fused_pats.size();
+    const uint8_t *src = s->d_tmp;
+    int64_t n = (int64_t)s->fused_src_len;
+    for (int i = 0; i < count; ++i) {
+        const int b = i & 1;
+        auto dst_for = [&](int64_t need, uint8_t **dst) -> int {
+            hipError_t e = grow((void **)&s->d_q[b], &s->cap_q[b], (size_t)(need > 0 ? need : 1), false, s->stream);
+            if (e != hipSuccess)
+                return (int)e;
+            *dst = s->d_q[b];
+            return 0;
+        };
+        PassResult res;
+        int mode = 0;
+        ReplaceParams qp{};
+        const std::vector<uint8_t> &P = s->fused_pats[(size_t)i], &R = s->fused_reps[(size_t)i];
+        int rc = run_pass(s, src, n, P.data(), (int)P.size(), R.data(), (int)R.size(), 0, -1, false, dst_for, &res,
+                          nullptr, &mode, &qp);
+        if (rc != 0)
+            return rc;
+        if (i == count - 1) {
+            s->q_params = qp;
+            s->q_params.src = src;
+            s->q_params.n = n;
+            s->q_mode = mode;
+            s->q_valid = true;
+        }
+        src = s->d_q[b];
+        n = res.out_len;
+    }
+    s->fused_pending = false;
     return 0;
 }
 
@@ -844,6 +1075,11 @@ void destroy(Session *s)
     for (uint8_t *p : {s->h_sin[0], s->h_sin[1]})
         if (p)
             (void)hipHostFree(p);
+    for (void *p : {(void *)s->d_fused, (void *)s->d_q[0], (void *)s->d_q[1]})
+        if (p)
+            (void)hipFree(p);
+    if (s->h_fused)
+        (void)hipHostFree(s->h_fused);
     for (hipEvent_t e : {s->ev_in[0], s->ev_in[1], s->ev_out[0], s->ev_out[1]})
         if (e)
             (void)hipEventDestroy(e);
@@ -920,6 +1156,7 @@ int cr_stream_replace(void *h, const uint8_t *src, size_t n, const uint8_t *cons
     *out_host = nullptr;
     *out_len = 0;
     s->q_valid = false;   // the pipeline's passes are not the session's applies
+    s->fused_pending = false;
     return session_stream(s, src, n, pats, pat_lens, reps, rep_lens, count, chunk_bytes, out_host, out_len, stats);
 }
 
@@ -1042,6 +1279,14 @@ int cuda_replace_query_prefix(void *h, size_t limit, int *kept_before_limit, int
 int cuda_replace_apply_batch(void *h, const uint8_t **pats, const int *pat_lens, const uint8_t **reps,
                              const int *rep_lens, int count, size_t *new_len)
 {
+    return cr_apply_batch_ex(h, pats, pat_lens, reps, rep_lens, count, new_len, 1, nullptr);
+}
+
+int cr_apply_batch_ex(void *h, const uint8_t *const *pats, const int *pat_lens, const uint8_t *const *reps,
+                      const int *rep_lens, int count, size_t *new_len, int allow_fuse, int *fused)
+{
+    if (fused)
+        *fused = 0;
     if (!h)
         return -1;
     Session *s = (Session *)h;
@@ -1049,20 +1294,25 @@ int cuda_replace_apply_batch(void *h, const uint8_t **pats, const int *pat_lens,
     if (!s->alive)
         return -3;
     DeviceGuard g(s->device);
-    for (int i = 0; i < count; ++i) {
-        if (!pats || !pat_lens)
-            break;   // no patterns: every pair is skipped
-        const uint8_t *p = pats[i];
-        const int pl = pat_lens[i];
-        const uint8_t *r = (reps && rep_lens) ? reps[i] : nullptr;
-        const int rl = (reps && rep_lens) ? rep_lens[i] : 0;
-        int rc = session_apply(s, p, pl, r, rl, 0, nullptr, nullptr);
+    if (pats && pat_lens && count > 0) {   // no patterns: every pair is skipped
+        int rc = session_apply_batch(s, pats, pat_lens, reps, rep_lens, count, allow_fuse != 0, fused);
         if (rc != 0)
             return rc;
     }
     if (new_len)
         *new_len = s->len;
     return 0;
+}
+
+int cr_batch_fusable(const uint8_t *const *pats, const int *pat_lens, const uint8_t *const *reps, const int *rep_lens,
+                     int count)
+{
+    FusedTable *t = new (std::nothrow) FusedTable;
+    if (!t)
+        return -2;
+    const bool ok = build_fused(pats, pat_lens, reps, rep_lens, count, t);
+    delete t;
+    return ok ? 1 : 0;
 }
 
 int cuda_replace_result(void *h, uint8_t **out_host, size_t *outlen_host)

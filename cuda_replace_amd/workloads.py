@@ -20,6 +20,11 @@ Workloads (sizes are the full benchmark sizes; tests pass smaller ``n``):
   positions (so they occur, some self-overlapping), replacements 0-32 seeded
   bytes of ``A-Z0-9``; a pair whose replacement would grow the sample by more
   than 5% gets a replacement no longer than its pattern.
+* ``entity_pairs`` / ``entity_text`` -- the 128-pair session batch: patterns
+  ``b"&" + 3-6 seeded lowercase letters + b";"`` (distinct), replacements 1-4
+  seeded bytes of ``A-Z0-9``, applied in order; the text gets non-overlapping
+  pattern occurrences starting at ~0.5% of its positions (pattern uniform over
+  the list) and NUL bytes at ~0.01% of them.
 """
 from __future__ import annotations
 
@@ -150,3 +155,39 @@ def uneven_documents(num_docs: int = 8192, median: int = 64 * KiB, sigma: float 
     counts = rng.integers(1, max_pairs + 1, size=num_docs)
     pairs = [[pool[int(k)] for k in rng.integers(0, pool_size, size=int(c))] for c in counts]
     return docs, pairs
+
+
+def entity_pairs(count: int = 128, seed: int = 3):
+    """``count`` distinct (entity pattern, replacement) pairs (see module docstring)."""
+    rng = np.random.default_rng([seed, 0xE471])
+    seen, pairs = set(), []
+    while len(pairs) < count:
+        k = int(rng.integers(3, 7))
+        pat = b"&" + rng.integers(ord("a"), ord("z") + 1, size=k, dtype=np.uint8).tobytes() + b";"
+        if pat in seen:
+            continue
+        seen.add(pat)
+        r = int(rng.integers(1, 5))
+        pairs.append((pat, _REP_ALPHABET[rng.integers(0, len(_REP_ALPHABET), size=r)].tobytes()))
+    return pairs
+
+
+def entity_text(n: int, pairs, density: float = 0.005, nul_density: float = 0.0001, seed: int = 3,
+                text_seed: int = 2) -> bytes:
+    """``n`` bytes of the text with the pairs' patterns and NULs planted (see module docstring)."""
+    buf = np.frombuffer(zipf_text(n, text_seed), dtype=np.uint8).copy()
+    rng = np.random.default_rng([seed, 0x7E47E])
+    nuls = rng.integers(0, max(1, n), size=int(n * nul_density))
+    buf[nuls] = 0
+    step = int(round(1 / density))
+    longest = max(len(p) for p, _ in pairs)
+    if n >= step + longest:
+        # one occurrence per step-byte slot, at a seeded offset that keeps it inside the slot
+        starts = np.arange(0, n - step - longest + 1, step, dtype=np.int64)
+        starts += rng.integers(0, step - longest, size=len(starts))
+        which = rng.integers(0, len(pairs), size=len(starts))
+        for k, (pat, _) in enumerate(pairs):
+            at = starts[which == k]
+            if len(at):
+                buf[at[:, None] + np.arange(len(pat))[None, :]] = np.frombuffer(pat, dtype=np.uint8)
+    return buf.tobytes()

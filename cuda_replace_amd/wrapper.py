@@ -44,6 +44,7 @@ REQUIRED_SYMBOLS = (
     "cr_session_length", "cr_query_prefix64", "cr_replace_device", "cr_pattern_mode",
     "cr_batch_create", "cr_batch_run", "cr_batch_destroy", "cr_stream_replace",
     "cr_server_create", "cr_server_submit", "cr_server_wait", "cr_server_release", "cr_server_destroy",
+    "cr_apply_batch_ex", "cr_batch_fusable",
 )
 
 
@@ -64,6 +65,19 @@ def _bytes_at(ptr, n: int) -> bytes:
 def _u8(b: bytes):
     """Pointer to the bytes object's own buffer (no copy; NULs allowed)."""
     return C.cast(C.c_char_p(b), c_uint8_p)
+
+
+def _pair_arrays(pairs):
+    """(pats, pat_lens, reps, rep_lens, keepalive) C arrays for a pair list."""
+    count = len(pairs)
+    bp = [bytes(p) for p, _ in pairs]
+    br = [bytes(r) for _, r in pairs]
+    pats = (c_uint8_p * count)(*[_u8(p) for p in bp])
+    reps = (c_uint8_p * count)(*[_u8(r) for r in br])
+    pat_lens = (C.c_int * count)(*[len(p) for p in bp])
+    rep_lens = (C.c_int * count)(*[len(r) for r in br])
+    return (C.cast(pats, c_uint8_pp), C.cast(pat_lens, c_int_p), C.cast(reps, c_uint8_pp),
+            C.cast(rep_lens, c_int_p), (pats, reps, pat_lens, rep_lens, bp, br))
 
 
 def _load(path: str) -> C.CDLL:
@@ -96,6 +110,11 @@ class CudaReplaceLib:
         lib.cuda_replace_apply_batch.argtypes = [c_void_p, c_uint8_pp, c_int_p, c_uint8_pp, c_int_p,
                                                  C.c_int, c_size_t_p]
         lib.cuda_replace_apply_batch.restype = C.c_int
+        lib.cr_apply_batch_ex.argtypes = [c_void_p, c_uint8_pp, c_int_p, c_uint8_pp, c_int_p, C.c_int, c_size_t_p,
+                                          C.c_int, c_int_p]
+        lib.cr_apply_batch_ex.restype = C.c_int
+        lib.cr_batch_fusable.argtypes = [c_uint8_pp, c_int_p, c_uint8_pp, c_int_p, C.c_int]
+        lib.cr_batch_fusable.restype = C.c_int
         lib.cuda_replace_result.argtypes = [c_void_p, C.POINTER(c_uint8_p), c_size_t_p]
         lib.cuda_replace_result.restype = C.c_int
         lib.cuda_replace_close.argtypes = [c_void_p]
@@ -150,6 +169,17 @@ class CudaReplaceLib:
 
     def pattern_mode(self, pat: bytes) -> int:
         return int(self.lib.cr_pattern_mode(_u8(pat), len(pat)))
+
+    def batch_fusable(self, pairs: Sequence[Tuple[bytes, bytes]]) -> bool:
+        """True if the library would run these pairs as one fused pass (host check
+        only, no GPU call)."""
+        if not pairs:
+            return False
+        arrs = _pair_arrays(pairs)
+        rc = int(self.lib.cr_batch_fusable(*arrs[:4], len(pairs)))
+        if rc < 0:
+            raise RuntimeError(f"cr_batch_fusable failed rc={rc}")
+        return rc == 1
 
     def unified(self, src: bytes, pat: bytes, rep: bytes) -> bytes:
         """One-shot replace (py:124-141)."""
@@ -212,21 +242,22 @@ class Session:
             raise RuntimeError(f"cuda_replace_apply_seeded failed rc={rc}")
         return int(new_len.value), int(last_rel.value)
 
-    def apply_batch(self, pairs: Sequence[Tuple[bytes, bytes]]) -> int:
+    def apply_batch(self, pairs: Sequence[Tuple[bytes, bytes]], fuse: bool = True) -> int:
+        """Applies the pairs in order.  With fuse (the default, as in
+        cuda_replace_apply_batch) pairs that cr_batch_fusable accepts run as one
+        pass; self.last_batch_fused says whether they did."""
         count = len(pairs)
+        self.last_batch_fused = False
         if count == 0:
             return self.length()
-        pats = (c_uint8_p * count)(*[_u8(p) for p, _ in pairs])
-        reps = (c_uint8_p * count)(*[_u8(r) for _, r in pairs])
-        pat_lens = (C.c_int * count)(*[len(p) for p, _ in pairs])
-        rep_lens = (C.c_int * count)(*[len(r) for _, r in pairs])
-        keep = [p for p, _ in pairs] + [r for _, r in pairs]   # noqa: F841 (buffers stay alive)
+        arrs = _pair_arrays(pairs)
         new_len = c_size_t()
-        rc = self.lib.lib.cuda_replace_apply_batch(self.h, C.cast(pats, c_uint8_pp), C.cast(pat_lens, c_int_p),
-                                                   C.cast(reps, c_uint8_pp), C.cast(rep_lens, c_int_p),
-                                                   count, C.byref(new_len))
+        fused = C.c_int(0)
+        rc = self.lib.lib.cr_apply_batch_ex(self.h, *arrs[:4], count, C.byref(new_len), 1 if fuse else 0,
+                                            C.byref(fused))
         if rc != 0:
             raise RuntimeError(f"cuda_replace_apply_batch failed rc={rc}")
+        self.last_batch_fused = bool(fused.value)
         return int(new_len.value)
 
     def query_prefix(self, limit: int) -> Tuple[int, int]:

@@ -84,8 +84,10 @@ CR_API int cuda_replace_query_prefix(void *h, size_t limit,
                               int *kept_before_limit, int *last_kept_before_limit);
 
 /* count sequential seedless applies.  NULL pats or pat_lens skips every pair;
- * NULL reps or rep_lens means empty replacements.  Replaces
- * cuda_replace.cu:2465-2527. */
+ * NULL reps or rep_lens means empty replacements.  When cr_batch_fusable holds
+ * for the pairs, they run as one fused pass over the buffer (same result); a
+ * later prefix query then replays the pairs one by one to answer for the last
+ * pair.  Replaces cuda_replace.cu:2465-2527. */
 CR_API int cuda_replace_apply_batch(void *h, const uint8_t **pats, const int *pat_lens,
                              const uint8_t **reps, const int *rep_lens,
                              int count, size_t *new_len);
@@ -96,6 +98,23 @@ CR_API int cuda_replace_result(void *h, uint8_t **out_host, size_t *outlen_host)
 
 /* Closes a session; NULL is ignored.  Replaces cuda_replace.cu:2565-2577. */
 CR_API void cuda_replace_close(void *h);
+
+/* cuda_replace_apply_batch with the fusion choice exposed: allow_fuse=0 always
+ * applies pair by pair; *fused (optional) receives 1 if the pairs ran as one
+ * fused pass.  No counterpart in the reference. */
+CR_API int cr_apply_batch_ex(void *h, const uint8_t *const *pats, const int *pat_lens,
+                             const uint8_t *const *reps, const int *rep_lens, int count,
+                             size_t *new_len, int allow_fuse, int *fused);
+
+/* 1 if applying the pairs in order equals one fused pass that replaces every
+ * occurrence of every pattern: 2..256 pairs; patterns of 2..64 bytes sharing
+ * their first byte, each border-free (no proper prefix equals a suffix),
+ * distinct, none inside another, no proper suffix of one equal to a prefix of
+ * another; replacements of 1..64 bytes with no byte that occurs in any pattern;
+ * patterns and replacements together at most 16384 bytes.  0 otherwise, -2 on
+ * allocation failure.  No counterpart in the reference. */
+CR_API int cr_batch_fusable(const uint8_t *const *pats, const int *pat_lens,
+                            const uint8_t *const *reps, const int *rep_lens, int count);
 
 /* Library version string.  Replaces cuda_replace.cu:2251-2254. */
 CR_API const char *cuda_replace_version(void);

@@ -1,0 +1,160 @@
+"""Fused multi-pattern apply_batch parity on the GPU: when cr_batch_fusable accepts
+a pair list, cuda_replace_apply_batch runs it as one pass, and the result must
+equal applying the pairs in order with bytes.replace (oracle apply_batch) and
+the same session's pair-by-pair path (fuse=False).
+
+Covers sizes around the 8 KiB tile and 16 KiB group boundaries, occurrences
+planted across every such boundary (the emit's coverage carry and halo probe),
+dense inputs made only of occurrences, heavy expansion (the unstaged emit) and
+heavy shrink (negative totals), 64-byte patterns, 256 pairs with colliding hash
+keys, and prefix queries after a fused batch (answered for the last pair by a
+replay of the pairs).
+"""
+import random
+
+import pytest
+
+from oracle import replace_oracle as ro
+from cuda_replace_amd import Session, workloads
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [0, 1, 5, 100, 8191, 8192, 8193, 16383, 16384, 16385, 40000, 200000, 3 << 20]
+LETTERS = b"abcdefgh"
+REP_ALPHA = b"XYZ0123456789"
+
+
+def entity_list(rng, k, min_body=1, max_body=6, max_rep=6):
+    """k distinct patterns b"&" + body + b";" (body over LETTERS) with replacements
+    over REP_ALPHA: always fusable."""
+    seen, out = set(), []
+    while len(out) < k:
+        body = bytes(rng.choice(LETTERS) for _ in range(rng.randint(min_body, max_body)))
+        pat = b"&" + body + b";"
+        if pat in seen:
+            continue
+        seen.add(pat)
+        out.append((pat, bytes(rng.choice(REP_ALPHA) for _ in range(rng.randint(1, max_rep)))))
+    return out
+
+
+def planted(rng, n, pairs, density=0.02, boundaries=()):
+    buf = bytearray(rng.choice(LETTERS + b" &;\x00") for _ in range(n))
+    pats = [p for p, _ in pairs]
+    i = 0
+    while i < n:
+        if rng.random() < density:
+            p = rng.choice(pats)
+            if i + len(p) <= n:
+                buf[i:i + len(p)] = p
+                i += len(p)
+                continue
+        i += 1
+    for b in boundaries:   # an occurrence ending 1..L-1 bytes past a boundary
+        p = rng.choice(pats)
+        at = b - rng.randint(1, len(p) - 1)
+        if 0 <= at and at + len(p) <= n:
+            buf[at:at + len(p)] = p
+    return bytes(buf)
+
+
+def run(lib, src, pairs, fuse=True):
+    with Session(lib, src) as s:
+        n = s.apply_batch(pairs, fuse=fuse)
+        out = s.result()
+        assert n == len(out)
+        return out, s.last_batch_fused
+
+
+def check(lib, src, pairs):
+    assert lib.batch_fusable(pairs)
+    want = ro.apply_batch(src, pairs)
+    got, fused = run(lib, src, pairs)
+    assert fused
+    assert got == want, (len(src), pairs[:3])
+    seq, fused = run(lib, src, pairs, fuse=False)
+    assert not fused and seq == want
+    return got
+
+
+def test_fused_sizes_and_boundaries(lib):
+    rng = random.Random(41)
+    for n in SIZES:
+        pairs = entity_list(rng, rng.randint(2, 12))
+        bounds = [k * 8192 for k in range(1, n // 8192 + 1)][:600]
+        check(lib, planted(rng, n, pairs, boundaries=bounds), pairs)
+
+
+def test_fused_dense_occurrences_only(lib):
+    rng = random.Random(42)
+    for n_occ in (1, 3, 5000, 60000):
+        pairs = entity_list(rng, 6, max_body=3)
+        src = b"".join(rng.choice(pairs)[0] for _ in range(n_occ))
+        check(lib, src, pairs)
+
+
+def test_fused_heavy_expand_and_shrink(lib):
+    rng = random.Random(43)
+    # 2-byte patterns to 64-byte replacements: tiles grow ~32x, the emit writes directly
+    pairs = [(b"&" + bytes([c]), bytes(rng.choice(REP_ALPHA) for _ in range(64))) for c in b"abcd"]
+    src = b"".join(rng.choice([b"&a", b"&b", b"&c", b"&d", b"x"]) for _ in range(100000))
+    check(lib, src, pairs)
+    # 64-byte patterns to 1-byte replacements: the total length change is negative
+    pairs = [(b"&" + bytes([c]) * 62 + b";", b"Z") for c in b"abcdef"]
+    src = b"".join(rng.choice([p for p, _ in pairs] + [b"q"]) for _ in range(40000))
+    out = check(lib, src, pairs)
+    assert len(out) < len(src) // 10
+
+
+def test_fused_many_pairs_colliding_keys(lib):
+    rng = random.Random(44)
+    # 256 patterns sharing their first 5 bytes: one hash key, a 256-long probe chain
+    pairs = [(b"&abcd" + bytes([97 + i // 16, 97 + i % 16]) + b";", bytes([48 + i % 10, 65 + i // 10 % 26]))
+             for i in range(256)]
+    src = planted(rng, 300000, pairs, density=0.05)
+    check(lib, src, pairs)
+    pairs = entity_list(rng, 256, min_body=2)
+    check(lib, planted(rng, 500000, pairs, density=0.05), pairs)
+
+
+def test_query_after_fused_batch(lib):
+    rng = random.Random(45)
+    pairs = entity_list(rng, 5)
+    src = planted(rng, 100000, pairs, density=0.05)
+    mid = ro.apply_batch(src, pairs[:-1])
+    last_pat = pairs[-1][0]
+    limits = [0, 1, 4000, 50000, len(mid), len(mid) + 10]
+    with Session(lib, src) as s:
+        s.apply_batch(pairs)
+        assert s.last_batch_fused
+        for lim in limits:
+            assert s.query_prefix(lim) == ro.query_prefix(mid, last_pat, None, min(lim, len(mid))), lim
+        assert s.result() == ro.apply_batch(src, pairs)
+        # a plain apply after the fused batch: queries follow it
+        out = s.result()
+        s.apply(b"XY", b"")
+        for lim in (0, 777, len(out)):
+            assert s.query_prefix(lim) == ro.query_prefix(out, b"XY", None, min(lim, len(out)))
+        assert s.result() == out.replace(b"XY", b"")
+
+
+def test_fused_then_sequential_batches_on_one_session(lib):
+    rng = random.Random(46)
+    src = planted(rng, 70000, entity_list(rng, 4), density=0.03)
+    with Session(lib, src) as s:
+        want = src
+        for _ in range(4):
+            pairs = entity_list(rng, rng.randint(2, 6))
+            s.apply_batch(pairs)
+            assert s.last_batch_fused
+            want = ro.apply_batch(want, pairs)
+            s.apply_batch([(b"X", b"xx"), (b"xx", b"Q")])   # not fusable: pair by pair
+            assert not s.last_batch_fused
+            want = ro.apply_batch(want, [(b"X", b"xx"), (b"xx", b"Q")])
+            assert s.result() == want
+
+
+def test_fused_entity_workload(lib):
+    pairs = workloads.entity_pairs()
+    src = workloads.entity_text(64 << 20, pairs)
+    check(lib, src, pairs)
