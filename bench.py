@@ -28,6 +28,13 @@ serving  = open-loop Poisson arrivals (1000 requests/s for --serve-seconds) of
 streaming = the text repeated to --stream-gib GiB, streamed through the pipelined
            chunked path in 256 MiB chunks (fragment -> 12 bytes), host to host,
            checked against bytes.replace of the 1 GiB piece.
+session_batch = cuda_replace_apply_batch with 128 entity pairs (b"&" + 3-6 letters
+           + b";" -> 1-4 bytes of A-Z0-9, applied in order) on a resident
+           --session-gib GiB buffer of the text with the patterns planted at 0.5%
+           of positions and NULs at 0.01%: the fused single pass (default) and the
+           pair-by-pair path on the same session, each timed host-side around the
+           call; both outputs checked against the oracle (128 bytes.replace passes
+           per space-delimited slice, on a fork pool before the GPU is touched).
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--size-mib 1024]
 """
@@ -94,6 +101,97 @@ def cpu_parallel(key: str, pat: bytes, rep: bytes, procs: int):
         out = b"".join(pieces)
         dt = time.perf_counter() - t
     return out, dt, len(cuts) - 1
+
+
+_SB = {}
+
+
+def _sb_piece(args):
+    a, b = args
+    from oracle import replace_oracle as ro
+    return ro.apply_batch(_SB["src"][a:b], _SB["pairs"])
+
+
+def session_batch_prepare(ro, gib: float, procs: int) -> dict:
+    """Input, expected output and single-thread CPU sample of the session batch.
+    Runs before the GPU is initialised (it forks).  Slices are cut just after a
+    space: no pattern holds a space and no pass removes one, so every pass's
+    occurrences stay inside a slice and the slices' results concatenate exactly."""
+    from cuda_replace_amd import workloads
+    pairs = workloads.entity_pairs()
+    n = int(gib * (1 << 30))
+    t0 = time.time()
+    src = workloads.entity_text(n, pairs)
+    log(f"[bench] session batch input: {n / 2**30:.2f} GiB, {len(pairs)} pairs, generated in "
+        f"{time.time() - t0:.1f}s")
+    assert all(b" " not in p for p, _ in pairs)
+    _SB["src"], _SB["pairs"] = src, pairs
+    parts = max(procs * 4, 1)
+    cuts = [0]
+    for k in range(1, parts):
+        c = src.find(b" ", n * k // parts)
+        if c != -1 and c + 1 > cuts[-1]:
+            cuts.append(c + 1)
+    cuts.append(n)
+    t = time.perf_counter()
+    with mp.get_context("fork").Pool(procs) as pool:
+        expected = b"".join(pool.map(_sb_piece, [(cuts[i], cuts[i + 1]) for i in range(len(cuts) - 1)]))
+    log(f"[bench] session batch oracle: {time.perf_counter() - t:.1f}s on {procs} processes")
+    sample_n = src.find(b" ", min(n, 64 << 20)) + 1 or n
+    t = time.perf_counter()
+    ro.apply_batch(src[:sample_n], pairs)
+    cpu_s = time.perf_counter() - t
+    _SB.clear()
+    return {"src": src, "pairs": pairs, "expected": expected, "cpu_sample_bytes": sample_n, "cpu_s": cpu_s,
+            "oracle_procs": procs}
+
+
+def session_batch_bench(lib, prep: dict, steps: int) -> dict:
+    """Fused vs pair-by-pair cuda_replace_apply_batch on one resident session."""
+    from cuda_replace_amd import Session
+    src, pairs, expected = prep["src"], prep["pairs"], prep["expected"]
+    n = len(src)
+    fusable = lib.batch_fusable(pairs)
+    with Session(lib, src) as s:
+        s.apply_batch(pairs)                      # grows the session's buffers
+        fused_t = []
+        for _ in range(steps):
+            s.reset(src)
+            t = time.perf_counter()
+            out_len = s.apply_batch(pairs)
+            fused_t.append(time.perf_counter() - t)
+        fused = s.last_batch_fused
+        exact_fused = out_len == len(expected) and s.result() == expected
+        s.reset(src)
+        t = time.perf_counter()
+        s.apply_batch(pairs, fuse=False)
+        seq_t = time.perf_counter() - t
+        exact_seq = s.result() == expected
+    f_ms = statistics.median(fused_t) * 1e3
+    log(f"[bench] session batch: fused={fused} {f_ms:.2f} ms, pair by pair {seq_t * 1e3:.1f} ms, "
+        f"bit_exact fused={exact_fused} sequential={exact_seq}")
+    algo = n + len(expected)
+    return {
+        "workload": f"{n / 2**30:.0f} GiB resident text, 128 pairs b'&'+3-6 letters+b';' -> 1-4 B of A-Z0-9 in "
+                    f"order, patterns at 0.5% of positions, NUL at 0.01%",
+        "input_bytes": n,
+        "output_bytes": len(expected),
+        "fusable": fusable,
+        "fused": fused,
+        "fused_ms": round(f_ms, 3),
+        "fused_input_GBps": round(n / (f_ms / 1e3) / 1e9, 1),
+        "fused_hbm_frac": round(algo / (f_ms / 1e3) / HBM_PEAK, 4),
+        "pair_by_pair_ms": round(seq_t * 1e3, 2),
+        "pair_by_pair_input_GBps": round(n / seq_t / 1e9, 2),
+        "speedup": round(seq_t * 1e3 / f_ms, 1),
+        "timing": "host wall time around cuda_replace_apply_batch (table upload, count, spine, length "
+                  "readback, emit, stream sync), median of the timed calls",
+        "cpu_baseline": {"value": round(prep["cpu_sample_bytes"] / prep["cpu_s"] / 1e9, 4), "unit": "GB/s",
+                         "cores": 1, "kind": "port",
+                         "sample": f"first {prep['cpu_sample_bytes'] / 2**20:.0f} MiB (cut after a space), "
+                                   f"128 bytes.replace passes, single thread"},
+        "bit_exact": bool(exact_fused and exact_seq),
+    }
 
 
 # ------------------------------------------------------------------ main
@@ -259,6 +357,8 @@ def main() -> int:
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming measurement")
     ap.add_argument("--stream-gib", type=int, default=16)
+    ap.add_argument("--no-session-batch", action="store_true", help="skip the 128-pair session batch")
+    ap.add_argument("--session-gib", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the replace path runs on a single MI355X "
@@ -300,6 +400,7 @@ def main() -> int:
             assert out == expected[name], name
             par[name] = (dt, pieces)
             log(f"[bench] cpu parallel {name}: {dt:.3f}s over {pieces} piece(s)")
+    sb_prep = None if args.no_session_batch else session_batch_prepare(ro, args.session_gib, procs)
 
     import torch
     dev = torch.device("cuda:0")
@@ -373,6 +474,8 @@ def main() -> int:
     lib.lib.cuda_replace_close(h)
     batched = None if args.no_batch else batch_bench(lib, ro, args.batch_docs)
     serving = None if args.no_serve else serve_bench(lib, ro, text, args.serve_seconds, workers=args.serve_workers)
+    session_batch = None if sb_prep is None else session_batch_bench(lib, sb_prep, max(3, args.steps))
+    sb_prep = None
     streaming = None if args.no_stream else stream_bench(lib, text, frag, rep_e, expected["text_expand"],
                                                          args.stream_gib)
 
@@ -436,10 +539,11 @@ def main() -> int:
         "batched": batched,
         "serving": serving,
         "streaming": streaming,
+        "session_batch": session_batch,
         "detail": detail,
     }
     print(json.dumps(line), flush=True)
-    ok = exact and all(x is None or x["bit_exact"] for x in (batched, serving, streaming))
+    ok = exact and all(x is None or x["bit_exact"] for x in (batched, serving, streaming, session_batch))
     return 0 if ok else 1
 
 
