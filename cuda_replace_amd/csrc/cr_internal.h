@@ -72,18 +72,31 @@ constexpr int kFusedPool = 16384;      // pattern + replacement bytes
 constexpr int kFusedHashBits = 10;
 constexpr int kFusedSlots = 1 << kFusedHashBits;
 
+// slot entry: key (bytes 1..key_len of the pattern, little-endian) in bits 0-31,
+// pattern index in 32-39, pattern length in 40-47, replacement length in 48-55,
+// bit 63 set; 0 = empty.  One load gives everything but the pattern's bytes.
+constexpr uint64_t kFusedUsed = 1ull << 63;
+
 struct FusedTable {
     int32_t count;
     int32_t key_len;                   // pattern bytes after the first one that are hashed (1..4)
     int32_t min_len;
     int32_t max_len;
-    uint16_t slot[kFusedSlots];        // pattern index + 1, 0 = empty; linear probing
+    uint32_t key_mask;                 // low key_len bytes set
+    uint32_t pad[3];
+    uint64_t slot[kFusedSlots];        // linear probing
+    uint4 head[kMaxFusedPats];         // first 16 bytes of each pattern, zero padded
+    uint4 rep_head[kMaxFusedPats];     // first 16 bytes of each replacement, zero padded
     uint16_t pat_len[kMaxFusedPats];
     uint16_t rep_len[kMaxFusedPats];
     uint16_t pat_off[kMaxFusedPats];   // into pool
     uint16_t rep_off[kMaxFusedPats];
-    uint8_t pool[kFusedPool];
+    uint8_t pool[kFusedPool + 16];     // 16 bytes of slack: pattern bytes are compared 16 at a time
 };
+
+__host__ __device__ inline int fused_entry_index(uint64_t e) { return (int)((e >> 32) & 0xff); }
+__host__ __device__ inline int fused_entry_pat_len(uint64_t e) { return (int)((e >> 40) & 0xff); }
+__host__ __device__ inline int fused_entry_rep_len(uint64_t e) { return (int)((e >> 48) & 0xff); }
 
 // Hash of the key bytes after the shared first byte (multiplicative, top bits).
 __host__ __device__ inline uint32_t fused_hash(uint32_t key)

@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(kSpineThreads)
 spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk, int last_chunk)
 {
     constexpr int W = MODE == kModeTable ? LT : 1;   // words per group summary
-    constexpr int V = MODE == kModeFused ? 8 : SpineShape<W>::V;
+    constexpr int V = SpineShape<W>::V;
     constexpr int64_t kBlock = (int64_t)kWave * V;   // groups per wave and pass
     constexpr int64_t kPass = kBlock * kSpineWaves;
     __shared__ uint64_t s_wt[kSpineWaves][LT];
@@ -530,20 +530,21 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
     for (int64_t base = 0; base < groups; base += kPass) {
         const int64_t wb = base + (int64_t)wave * kBlock;   // wave's first group (chunk-relative)
         if constexpr (MODE == kModeFused) {
-            // signed length changes (replacement - pattern bytes) per group
-            int64_t v[V], ex[V];
-            int64_t run = 0;
+            // signed length changes (replacement - pattern bytes) per group: at most
+            // 8192 occurrences x 63 bytes, so a wave's V * 64 groups sum within int32
+            int32_t v[V], ex[V];
+            int32_t run = 0;
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int64_t gi = wb + (int64_t)j * kWave + lane;
-                v[j] = gi < groups ? (int64_t)(int32_t)p.gsum[first_group + gi] : 0;
+                v[j] = gi < groups ? (int32_t)p.gsum[first_group + gi] : 0;
             }
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                int64_t inc = v[j];
+                int32_t inc = v[j];
 #pragma unroll
                 for (int d = 1; d < kWave; d <<= 1) {
-                    const int64_t u = __shfl_up(inc, d, kWave);
+                    const int32_t u = __shfl_up(inc, d, kWave);
                     if (lane >= d)
                         inc += u;
                 }
@@ -1114,38 +1115,55 @@ batch_spine_kernel(BatchArgs a)
 }
 
 // ---- fused multi-pattern pass ------------------------------------------------
-// Index of the pattern occurring at sp (LDS; avail source bytes from there), or -1.
-// The caller has matched the shared first byte; the key is the next key_len bytes.
-__device__ __forceinline__ int fused_probe(const FusedTable *t, const uint8_t *sp, int64_t avail)
+// Bytes [0, len) of a and b equal (len 0..16).
+__device__ __forceinline__ bool head_equal(uint4 a, uint4 b, int len)
+{
+    uint32_t d = 0;
+    const uint32_t x[4] = {a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w};
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int l = len - 4 * w;
+        const uint32_t m = l >= 4 ? 0xffffffffu : (l <= 0 ? 0u : (1u << (8 * l)) - 1u);
+        d |= x[w] & m;
+    }
+    return d == 0;
+}
+
+// Slot entry of the pattern occurring at sp (LDS; avail source bytes from there),
+// or 0.  The caller has matched the shared first byte; the key is the next key_len
+// bytes.  Two dependent table loads (slot, then the pattern's first 16 bytes) for
+// patterns of up to 16 bytes; longer ones compare the rest 16 bytes at a time.
+__device__ __forceinline__ uint64_t fused_probe(const FusedTable *t, const uint8_t *sp, int64_t avail)
 {
     if (avail < t->min_len)
-        return -1;
-    const int kl = t->key_len;
-    uint32_t key = 0;
-    for (int b = 0; b < kl; ++b)
-        key |= (uint32_t)sp[1 + b] << (8 * b);
+        return 0;
+    uint32_t key;
+    __builtin_memcpy(&key, sp + 1, 4);
+    key &= t->key_mask;
+    const uint4 text = lds_load16(sp);
     uint32_t h = fused_hash(key);
     for (int probe = 0; probe < kFusedSlots; ++probe) {
-        const int sl = t->slot[h];
-        if (sl == 0)
-            return -1;
-        const int k = sl - 1;
-        const int L = t->pat_len[k];
-        if (L <= avail) {
-            const uint8_t *pp = t->pool + t->pat_off[k];
-            bool ok = true;
-            for (int i = 1; i < L; ++i) {
-                if (sp[i] != pp[i]) {
-                    ok = false;
-                    break;
+        const uint64_t e = t->slot[h];
+        if (e == 0)
+            return 0;
+        const int L = fused_entry_pat_len(e);
+        if ((uint32_t)e == key && L <= avail) {
+            const int k = fused_entry_index(e);
+            bool ok = head_equal(text, t->head[k], L < 16 ? L : 16);
+            if (ok && L > 16) {
+                const uint8_t *pp = t->pool + t->pat_off[k];
+                for (int i = 16; ok && i < L; i += 16) {
+                    uint4 pv;
+                    __builtin_memcpy(&pv, pp + i, 16);
+                    ok = head_equal(lds_load16(sp + i), pv, L - i < 16 ? L - i : 16);
                 }
             }
             if (ok)
-                return k;
+                return e;
         }
         h = (h + 1) & (kFusedSlots - 1);
     }
-    return -1;
+    return 0;
 }
 
 // Group summary: the signed length change sum(rep_len - pat_len) over every
@@ -1176,9 +1194,9 @@ __device__ __forceinline__ void fused_count_group(const ReplaceParams &p, const 
         while (m) {
             const int q = __builtin_ctz(m);
             m &= m - 1;
-            const int k = fused_probe(t, sp + q, p.n - (gseg + q));
-            if (k >= 0)
-                delta += (int32_t)t->rep_len[k] - (int32_t)t->pat_len[k];
+            const uint64_t e = fused_probe(t, sp + q, p.n - (gseg + q));
+            if (e)
+                delta += fused_entry_rep_len(e) - fused_entry_pat_len(e);
         }
     }
 #pragma unroll
@@ -1246,14 +1264,19 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
         uint32_t kept = 0;
         int d = 0;
         int e = kNegInf;
+        uint64_t e0 = 0, e1 = 0;   // entries of my first two occurrences (the emit reuses them)
+        int nk = 0;
         while (m) {
             const int q = __builtin_ctz(m);
             m &= m - 1;
-            const int k = fused_probe(t, sp + q, n - (start + seg + q));
-            if (k >= 0) {
+            const uint64_t en = fused_probe(t, sp + q, n - (start + seg + q));
+            if (en) {
                 kept |= 1u << q;
-                d += (int)t->rep_len[k] - (int)t->pat_len[k];
-                e = seg + q + (int)t->pat_len[k];
+                d += fused_entry_rep_len(en) - fused_entry_pat_len(en);
+                e = seg + q + fused_entry_pat_len(en);
+                e1 = nk == 1 ? en : e1;
+                e0 = nk == 0 ? en : e0;
+                ++nk;
             }
         }
         if (i == 0) {
@@ -1262,9 +1285,9 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
             if (ti > 0 && tid < t->max_len - 1) {
                 const int q = -1 - tid;
                 if (s_in[in_off + q] == first) {
-                    const int k = fused_probe(t, s_in + in_off + q, n - (start + q));
-                    if (k >= 0 && q + (int)t->pat_len[k] > 0)
-                        atomicMax(&s_halo_end, q + (int)t->pat_len[k]);
+                    const uint64_t en = fused_probe(t, s_in + in_off + q, n - (start + q));
+                    if (en && q + fused_entry_pat_len(en) > 0)
+                        atomicMax(&s_halo_end, q + fused_entry_pat_len(en));
                 }
             }
             __syncthreads();
@@ -1313,6 +1336,7 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
             const bool staged = out_len + 32 <= kStage;
             int64_t o = (seg + c_i - c_t) + d_loc;
             int j = c_i;
+            int wi = 0;
             uint32_t M = j >= 32 ? 0u : (kept & (0xffffffffu << j));
             while (j < jend) {
                 const int q = M ? __builtin_ctz(M) : jend;
@@ -1325,15 +1349,33 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
                 }
                 if (q >= jend)
                     break;
-                const int k = fused_probe(t, sp + q, n - (start + seg + q));
-                const int r = t->rep_len[k];
-                const uint8_t *rp = t->pool + t->rep_off[k];
-                if (staged)
-                    global_to_lds(s_out + o, rp, r);
-                else
-                    store_from_global(out_first + o, rp, r);
+                // occurrences are disjoint, so the loop meets my occurrences in order
+                const uint64_t en = wi == 0 ? e0 : (wi == 1 ? e1 : fused_probe(t, sp + q, n - (start + seg + q)));
+                ++wi;
+                const int r = fused_entry_rep_len(en);
+                const int k = fused_entry_index(en);
+                if (r <= 16) {
+                    const uint4 rv = t->rep_head[k];
+                    if (r == 16) {
+                        if (staged)
+                            lds_store16(s_out + o, rv);
+                        else
+                            __builtin_memcpy(out_first + o, &rv, 16);
+                    } else {
+                        if (staged)
+                            store_tail(s_out + o, rv, r);
+                        else
+                            store_tail(out_first + o, rv, r);
+                    }
+                } else {
+                    const uint8_t *rp = t->pool + t->rep_off[k];
+                    if (staged)
+                        global_to_lds(s_out + o, rp, r);
+                    else
+                        store_from_global(out_first + o, rp, r);
+                }
                 o += r;
-                j = q + (int)t->pat_len[k];
+                j = q + fused_entry_pat_len(en);
                 M = j >= 32 ? 0u : (M & (0xffffffffu << j));
             }
             if (staged) {

@@ -683,6 +683,7 @@ bool build_fused(const uint8_t *const *pats, const int *pat_lens, const uint8_t 
     memset(t, 0, sizeof(FusedTable));
     t->count = count;
     t->key_len = std::min(4, min_len - 1);
+    t->key_mask = t->key_len == 4 ? 0xffffffffu : (1u << (8 * t->key_len)) - 1u;
     t->min_len = min_len;
     t->max_len = max_len;
     uint32_t off = 0;
@@ -695,13 +696,16 @@ bool build_fused(const uint8_t *const *pats, const int *pat_lens, const uint8_t 
         t->rep_off[i] = (uint16_t)off;
         memcpy(t->pool + off, reps[i], (size_t)rep_lens[i]);
         off += (uint32_t)rep_lens[i];
+        memcpy(&t->head[i], pats[i], (size_t)std::min(16, pat_lens[i]));
+        memcpy(&t->rep_head[i], reps[i], (size_t)std::min(16, rep_lens[i]));
         uint32_t key = 0;
         for (int b = 0; b < t->key_len; ++b)
             key |= (uint32_t)pats[i][1 + b] << (8 * b);
         uint32_t h = fused_hash(key);
         while (t->slot[h] != 0)
             h = (h + 1) & (kFusedSlots - 1);
-        t->slot[h] = (uint16_t)(i + 1);
+        t->slot[h] = (uint64_t)key | (uint64_t)i << 32 | (uint64_t)pat_lens[i] << 40 |
+                     (uint64_t)rep_lens[i] << 48 | kFusedUsed;
     }
     return true;
 }

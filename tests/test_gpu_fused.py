@@ -158,3 +158,20 @@ def test_fused_entity_workload(lib):
     pairs = workloads.entity_pairs()
     src = workloads.entity_text(64 << 20, pairs)
     check(lib, src, pairs)
+
+
+def test_fused_mixed_lengths(lib):
+    # 2- to 64-byte patterns in one list: a 1-byte hash key, pattern bytes compared
+    # 16 at a time past the first 16.  b"&h" is fusable with the b"&...;" patterns
+    # because no body starts with h.
+    rng = random.Random(47)
+    pairs, seen = [(b"&h", b"H2")], set()
+    for L in list(range(3, 65)) + [17, 31, 33, 48, 63, 64]:
+        while True:
+            pat = b"&" + bytes(rng.choice(LETTERS[:7]) for _ in range(L - 2)) + b";"
+            if pat not in seen:
+                break
+        seen.add(pat)
+        pairs.append((pat, bytes(rng.choice(REP_ALPHA) for _ in range(rng.randint(1, 64)))))
+    for n in (1000, 70000, 1 << 20):
+        check(lib, planted(rng, n, pairs, density=0.03, boundaries=range(8192, n, 8192)), pairs)
