@@ -540,6 +540,7 @@ def main() -> int:
         "serving": serving,
         "streaming": streaming,
         "session_batch": session_batch,
+        "autotune": lib.autotune_report(),
         "detail": detail,
     }
     print(json.dumps(line), flush=True)

@@ -30,7 +30,7 @@ def command(out: str = LIB_PATH) -> list:
         hipcc(),
         f"--offload-arch={ARCH}",
         "-O3",
-        "-std=c++17",
+        "-std=c++20",
         "-fPIC",
         "-shared",
         # packed FP32 instructions are kept out of every kernel

@@ -263,7 +263,7 @@ int run_other(Engine *E, const DocState &ds, const uint8_t *pat, int L, const ui
     p.dst = dst;
     p.out_len = len;
     p.write_output = 1;
-    CK(launch_emit(p, kModeBitmap, 0, 0, groups, s));
+    CK(launch_emit(p, kModeBitmap, 0, kGroupTiles, 0, groups, s));
     *out_len = len;
     return 0;
 }
@@ -338,6 +338,7 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
     }
 
     CK(hipEventRecord(E->ev0, s));
+    const int gt = kGroupTiles;
     double algo_bytes = 0;
     std::vector<int> active;
     for (int j = 0; j < max_pairs; ++j) {
@@ -361,7 +362,7 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
             pool += align_up((size_t)pat_lens[q] + 16, 16) + align_up((size_t)rep_lens[q] + 32, 16);
             if (cls[a] < kClasses) {
                 const int64_t tiles = (d.len + kTile - 1) / kTile;
-                cls_groups[cls[a]] += (tiles + kGroupTiles - 1) / kGroupTiles;
+                cls_groups[cls[a]] += (tiles + gt - 1) / gt;
                 cls_docs[cls[a]] += 1;
             }
         }
@@ -411,7 +412,7 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
             const int c = cls[a];
             if (c < kClasses) {
                 const int64_t tiles = (d.len + kTile - 1) / kTile;
-                const int64_t ng = (tiles + kGroupTiles - 1) / kGroupTiles;
+                const int64_t ng = (tiles + gt - 1) / gt;
                 bd.group0 = g_fill[c] - g_off[c];
                 bd.ngroups = (int32_t)ng;
                 for (int64_t g = 0; g < ng; ++g)
@@ -454,7 +455,7 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
             if (cls_docs[c] == 0)
                 continue;
             const BatchArgs ba = args_for(c);
-            CK(launch_batch_count(ba, kClassMode[c], kClassLt[c], cls_groups[c], s));
+            CK(launch_batch_count(ba, kClassMode[c], kClassLt[c], gt, cls_groups[c], s));
             CK(launch_batch_spine(ba, kClassMode[c], kClassLt[c], cls_docs[c], s));
         }
         if (A > 0)
@@ -505,7 +506,7 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
         for (int c = 0; c < kClasses; ++c) {
             if (cls_docs[c] == 0)
                 continue;
-            CK(launch_batch_emit(args_for(c), kClassMode[c], kClassLt[c], cls_groups[c], s));
+            CK(launch_batch_emit(args_for(c), kClassMode[c], kClassLt[c], gt, cls_groups[c], s));
         }
         // ---- empty-pattern and bitmap-path documents, one at a time ------------
         // (after a sync: the loop edits descriptors the upload above reads)

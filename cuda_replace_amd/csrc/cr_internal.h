@@ -25,7 +25,8 @@ constexpr int kStage = kTile + kTile / 8 + 64; // LDS image of a tile's output (
 
 // A group is kGroupTiles consecutive tiles handled by one workgroup; a chunk is
 // a run of consecutive groups handled by one count, one spine and one emit launch.
-constexpr int kGroupTiles = 2;
+constexpr int kGroupTiles = 2;                    // default tiles per group (1, 2 or 4 may run)
+constexpr int kMaxGroupTiles = 4;
 constexpr int kGroupBytes = kGroupTiles * kTile;    // 16 KiB of source per workgroup
 
 // Stats words carry the apply's 16-bit epoch in their top bits, so a stale word
@@ -169,20 +170,23 @@ struct BatchArgs {
     int64_t *doc_kept;          // kept starts per document of the round
 };
 
-hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s);
+// gt: tiles per group (1, 2 or 4), the same for a round's count and emit
+hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int gt, int64_t groups, hipStream_t s);
 hipError_t launch_batch_spine(const BatchArgs &a, int mode, int table_size, int64_t docs, hipStream_t s);
-hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s);
+hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int gt, int64_t groups, hipStream_t s);
 
 hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, int64_t first_group,
                         int64_t groups, hipStream_t s);   // which: 0 count, 1 spine, 2 emit
 
 // ---- launchers (cr_kernels.hip) --------------------------------------------
 // One chunk of groups [first_group, first_group + groups): count, spine, emit.
-hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int64_t first_group,
+// gt: tiles per group (1, 2 or 4; the bitmap emit only kGroupTiles); first_group and
+// groups count groups of that size.
+hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int gt, int64_t first_group,
                         int64_t groups, hipStream_t s);
 hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_t first_group,
                         int64_t groups, int64_t chunk, int last_chunk, hipStream_t s);
-hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int64_t first_group,
+hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int gt, int64_t first_group,
                        int64_t groups, hipStream_t s);
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s);

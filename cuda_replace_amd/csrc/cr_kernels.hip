@@ -28,6 +28,8 @@
 //   otherwise: candidate bitmap, one-wave serial greedy, then the emit kernel scatters.
 #include "cr_internal.h"
 
+#include <type_traits>
+
 #include <limits.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -311,8 +313,12 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
 }
 
 // ---- group staging ----------------------------------------------------------
-// LDS image of a group: [kHalo bytes before | kGroupBytes | kHalo bytes after].
-constexpr int kInBytes = kHalo + kGroupBytes + kHalo;
+// LDS image of a group of GT tiles: [kHalo bytes before | GT tiles | kHalo bytes after].
+template <int GT>
+struct GroupShape {
+    static constexpr int kBytes = GT * kTile;
+    static constexpr int kIn = kHalo + kBytes + kHalo;
+};
 
 // Issues the group's loads straight into LDS with global_load_lds (no registers):
 // every wave streams 1 KiB per instruction of the body; wave 0 fetches the left
@@ -320,9 +326,11 @@ constexpr int kInBytes = kHalo + kGroupBytes + kHalo;
 // s_waitcnt + a barrier.  The source is a 16-byte aligned device buffer, so the
 // granule that holds the last valid byte is readable; bytes at or past n are never
 // matched or emitted.
+template <int GT>
 __device__ __forceinline__ void load_group(const ReplaceParams &p, int64_t t0, int tid, bool left_halo,
                                            uint8_t *s_in)
 {
+    constexpr int kGroupBytes = GroupShape<GT>::kBytes;
     typedef __attribute__((address_space(3))) void lds_void;
     const int64_t start = t0 * kTile;
     const int64_t avail = p.n - start;
@@ -416,10 +424,11 @@ __device__ __forceinline__ Table<LT> segment_table(uint32_t m, int L)
 // group's kept count is a popcount.  Self-overlapping pattern: the group's carry
 // table (carry in -> carry out, kept), composed tile by tile in position order.
 // Workgroups never wait on each other.
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, uint32_t *summary)
 {
-    __shared__ __align__(16) uint8_t s_in[kInBytes];
+    constexpr int kGroupTiles = GT;
+    __shared__ __align__(16) uint8_t s_in[GroupShape<GT>::kIn];
     __shared__ uint8_t s_pat[kMaxSinglePassPat];
     __shared__ uint32_t s_red[kWaves];
     __shared__ uint32_t s_wt[kWaves][LT];
@@ -432,7 +441,7 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
         return;
     for (int i = tid; i < p.L; i += kThreads)
         s_pat[i] = p.pat[i];
-    load_group(p, t0, tid, false, s_in);
+    load_group<GT>(p, t0, tid, false, s_in);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
@@ -480,13 +489,13 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
     }
 }
 
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 __global__ void __launch_bounds__(kThreads)
 count_kernel(ReplaceParams p, int64_t first_group)
 {
     constexpr int W = MODE == kModeTable ? LT : 1;
     const int64_t g = first_group + blockIdx.x;
-    count_group<MODE, LT>(p, g, p.gsum + g * W);
+    count_group<MODE, LT, GT>(p, g, p.gsum + g * W);
 }
 
 // ---- spine: exclusive kept state of every group of a chunk -------------------
@@ -709,10 +718,11 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
 // the serial greedy, and the kept starts from the bitmap.
 // g: the group within p's buffer; pre: its state (kept before it, carry into it),
 // unused in bitmap mode; per-tile records are written when p.info is set.
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, const ChunkState *pre)
 {
-    __shared__ __align__(16) uint8_t s_in[kInBytes];
+    constexpr int kGroupTiles = GT;
+    __shared__ __align__(16) uint8_t s_in[GroupShape<GT>::kIn];
     __shared__ __align__(16) uint8_t s_out[kStage];
     __shared__ uint8_t s_pat[kMaxSinglePassPat];
     __shared__ __align__(16) uint8_t s_rep[kRepLds + 32];
@@ -745,7 +755,7 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
         s_halo_last = kNegInf;
 
     // ---- kept starts before the group and carry into it (from the spine) --------
-    load_group(p, t0, tid, MODE == kModeBorderFree, s_in);
+    load_group<GT>(p, t0, tid, MODE == kModeBorderFree, s_in);
     int64_t k_run = 0;   // kept starts before the current tile
     int c_run = 0;       // carry into the current tile
     if (MODE != kModeBitmap) {
@@ -991,12 +1001,12 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
     }
 }
 
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 __global__ void __launch_bounds__(kThreads)
 emit_kernel(ReplaceParams p, int64_t first_group)
 {
     const int64_t g = first_group + blockIdx.x;
-    emit_group<MODE, LT>(p, g, p.chunks_pre + g);
+    emit_group<MODE, LT, GT>(p, g, p.chunks_pre + g);
 }
 
 // ---- batches of independent documents ---------------------------------------
@@ -1023,23 +1033,23 @@ __device__ __forceinline__ ReplaceParams batch_view(const BatchArgs &a, int doc)
     return p;
 }
 
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 __global__ void __launch_bounds__(kThreads)
 batch_count_kernel(BatchArgs a)
 {
     constexpr int W = MODE == kModeTable ? LT : 1;
     const BatchGroup bg = a.groups[blockIdx.x];
     const ReplaceParams p = batch_view(a, bg.doc);
-    count_group<MODE, LT>(p, bg.g, a.gsum + (int64_t)blockIdx.x * W);
+    count_group<MODE, LT, GT>(p, bg.g, a.gsum + (int64_t)blockIdx.x * W);
 }
 
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 __global__ void __launch_bounds__(kThreads)
 batch_emit_kernel(BatchArgs a)
 {
     const BatchGroup bg = a.groups[blockIdx.x];
     const ReplaceParams p = batch_view(a, bg.doc);
-    emit_group<MODE, LT>(p, bg.g, a.pre + blockIdx.x);
+    emit_group<MODE, LT, GT>(p, bg.g, a.pre + blockIdx.x);
 }
 
 // One wave per document: an ordered scan of the document's group summaries, 64
@@ -1171,7 +1181,7 @@ __device__ __forceinline__ uint64_t fused_probe(const FusedTable *t, const uint8
 __device__ __forceinline__ void fused_count_group(const ReplaceParams &p, const FusedTable *t, int64_t g,
                                                   int32_t *summary)
 {
-    __shared__ __align__(16) uint8_t s_in[kInBytes];
+    __shared__ __align__(16) uint8_t s_in[GroupShape<kGroupTiles>::kIn];
     __shared__ int32_t s_red[kWaves];
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -1179,7 +1189,7 @@ __device__ __forceinline__ void fused_count_group(const ReplaceParams &p, const 
     const int64_t t0 = g * kGroupTiles;
     if (t0 >= p.num_tiles)
         return;
-    load_group(p, t0, tid, false, s_in);
+    load_group<kGroupTiles>(p, t0, tid, false, s_in);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const int seg = tid * kBytesPerThread;
@@ -1226,7 +1236,7 @@ __device__ __forceinline__ void global_to_lds(uint8_t *dst, const uint8_t *src, 
 __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const FusedTable *t, int64_t g,
                                                  const ChunkState *pre)
 {
-    __shared__ __align__(16) uint8_t s_in[kInBytes];
+    __shared__ __align__(16) uint8_t s_in[GroupShape<kGroupTiles>::kIn];
     __shared__ __align__(16) uint8_t s_out[kStage];
     __shared__ int s_wsum[kWaves];
     __shared__ int s_wmax[kWaves];
@@ -1241,7 +1251,7 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
         return;
     if (tid == 0)
         s_halo_end = 0;
-    load_group(p, t0, tid, true, s_in);
+    load_group<kGroupTiles>(p, t0, tid, true, s_in);
     int64_t k_run = pre->k_base;   // length change before the current tile
     int c_run = 0;                 // leading bytes of the current tile covered by an earlier match
     __builtin_amdgcn_s_waitcnt(0);
@@ -1680,17 +1690,17 @@ presence_kernel(const uint8_t *src, int64_t n, int64_t chunk, uint64_t *masks)
 }
 
 // ---- launchers --------------------------------------------------------------
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 hipError_t count_t(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s)
 {
-    hipLaunchKernelGGL((count_kernel<MODE, LT>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
+    hipLaunchKernelGGL((count_kernel<MODE, LT, GT>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
     return hipGetLastError();
 }
 
-template <int MODE, int LT>
+template <int MODE, int LT, int GT>
 hipError_t emit_t(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s)
 {
-    hipLaunchKernelGGL((emit_kernel<MODE, LT>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
+    hipLaunchKernelGGL((emit_kernel<MODE, LT, GT>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
     return hipGetLastError();
 }
 
@@ -1703,27 +1713,52 @@ hipError_t spine_t(const ReplaceParams &p, int64_t first_group, int64_t groups, 
     return hipGetLastError();
 }
 
-hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int64_t first_group, int64_t groups,
-                        hipStream_t s)
+// Calls F::template run<MODE, LT, GT>() for a runtime (mode, table size, tiles per
+// group); bitmap mode exists for kGroupTiles only.
+template <class F>
+hipError_t dispatch_mode(int mode, int table_size, int gt, F &&f)
 {
-    if (groups <= 0)
-        return hipSuccess;
+    auto by_gt = [&](auto mode_c, auto lt_c) -> hipError_t {
+        constexpr int M = decltype(mode_c)::value, LT = decltype(lt_c)::value;
+        switch (gt) {
+        case 1:
+            return f.template operator()<M, LT, 1>();
+        case 2:
+            return f.template operator()<M, LT, 2>();
+        case 4:
+            return f.template operator()<M, LT, 4>();
+        default:
+            return hipErrorInvalidValue;
+        }
+    };
+    using BF = std::integral_constant<int, kModeBorderFree>;
+    using TB = std::integral_constant<int, kModeTable>;
     if (mode == kModeBorderFree)
-        return count_t<kModeBorderFree, 1>(p, first_group, groups, s);
+        return by_gt(BF{}, std::integral_constant<int, 1>{});
     if (mode != kModeTable)
         return hipErrorInvalidValue;
     switch (table_size) {
     case 2:
-        return count_t<kModeTable, 2>(p, first_group, groups, s);
+        return by_gt(TB{}, std::integral_constant<int, 2>{});
     case 4:
-        return count_t<kModeTable, 4>(p, first_group, groups, s);
+        return by_gt(TB{}, std::integral_constant<int, 4>{});
     case 8:
-        return count_t<kModeTable, 8>(p, first_group, groups, s);
+        return by_gt(TB{}, std::integral_constant<int, 8>{});
     case 16:
-        return count_t<kModeTable, 16>(p, first_group, groups, s);
+        return by_gt(TB{}, std::integral_constant<int, 16>{});
     default:
         return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int gt, int64_t first_group, int64_t groups,
+                        hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    return dispatch_mode(mode, table_size, gt, [&]<int M, int LT, int GT>() {
+        return count_t<M, LT, GT>(p, first_group, groups, s);
+    });
 }
 
 hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_t first_group, int64_t groups,
@@ -1749,29 +1784,17 @@ hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_
     }
 }
 
-hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int64_t first_group, int64_t groups,
+hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int gt, int64_t first_group, int64_t groups,
                        hipStream_t s)
 {
     if (groups <= 0)
         return hipSuccess;
-    if (mode == kModeBorderFree)
-        return emit_t<kModeBorderFree, 1>(p, first_group, groups, s);
     if (mode == kModeBitmap)
-        return emit_t<kModeBitmap, 1>(p, first_group, groups, s);
-    if (mode != kModeTable)
-        return hipErrorInvalidValue;
-    switch (table_size) {
-    case 2:
-        return emit_t<kModeTable, 2>(p, first_group, groups, s);
-    case 4:
-        return emit_t<kModeTable, 4>(p, first_group, groups, s);
-    case 8:
-        return emit_t<kModeTable, 8>(p, first_group, groups, s);
-    case 16:
-        return emit_t<kModeTable, 16>(p, first_group, groups, s);
-    default:
-        return hipErrorInvalidValue;
-    }
+        return gt == kGroupTiles ? emit_t<kModeBitmap, 1, kGroupTiles>(p, first_group, groups, s)
+                                 : hipErrorInvalidValue;
+    return dispatch_mode(mode, table_size, gt, [&]<int M, int LT, int GT>() {
+        return emit_t<M, LT, GT>(p, first_group, groups, s);
+    });
 }
 
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s)
@@ -1838,53 +1861,35 @@ hipError_t launch_presence_index(const uint8_t *src, int64_t n, int64_t chunk, u
 }
 
 // ---- batch launchers ----------------------------------------------------------
-template <int MODE, int LT>
-hipError_t batch_t(const BatchArgs &a, int which, int64_t n, hipStream_t s)
+hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int gt, int64_t groups, hipStream_t s)
 {
-    if (which == 0)
-        hipLaunchKernelGGL((batch_count_kernel<MODE, LT>), dim3((unsigned)n), dim3(kThreads), 0, s, a);
-    else if (which == 1)
-        hipLaunchKernelGGL((batch_spine_kernel<MODE, LT>), dim3((unsigned)n), dim3(kWave), 0, s, a);
-    else
-        hipLaunchKernelGGL((batch_emit_kernel<MODE, LT>), dim3((unsigned)n), dim3(kThreads), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t batch_dispatch(const BatchArgs &a, int mode, int table_size, int which, int64_t n, hipStream_t s)
-{
-    if (n <= 0)
+    if (groups <= 0)
         return hipSuccess;
-    if (mode == kModeBorderFree)
-        return batch_t<kModeBorderFree, 1>(a, which, n, s);
-    if (mode != kModeTable)
-        return hipErrorInvalidValue;
-    switch (table_size) {
-    case 2:
-        return batch_t<kModeTable, 2>(a, which, n, s);
-    case 4:
-        return batch_t<kModeTable, 4>(a, which, n, s);
-    case 8:
-        return batch_t<kModeTable, 8>(a, which, n, s);
-    case 16:
-        return batch_t<kModeTable, 16>(a, which, n, s);
-    default:
-        return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s)
-{
-    return batch_dispatch(a, mode, table_size, 0, groups, s);
+    return dispatch_mode(mode, table_size, gt, [&]<int M, int LT, int GT>() {
+        hipLaunchKernelGGL((batch_count_kernel<M, LT, GT>), dim3((unsigned)groups), dim3(kThreads), 0, s, a);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_batch_spine(const BatchArgs &a, int mode, int table_size, int64_t docs, hipStream_t s)
 {
-    return batch_dispatch(a, mode, table_size, 1, docs, s);
+    if (docs <= 0)
+        return hipSuccess;
+    // the spine reads group summaries only, so one instantiation serves every group size
+    return dispatch_mode(mode, table_size, kGroupTiles, [&]<int M, int LT, int GT>() {
+        hipLaunchKernelGGL((batch_spine_kernel<M, LT>), dim3((unsigned)docs), dim3(kWave), 0, s, a);
+        return hipGetLastError();
+    });
 }
 
-hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int64_t groups, hipStream_t s)
+hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int gt, int64_t groups, hipStream_t s)
 {
-    return batch_dispatch(a, mode, table_size, 2, groups, s);
+    if (groups <= 0)
+        return hipSuccess;
+    return dispatch_mode(mode, table_size, gt, [&]<int M, int LT, int GT>() {
+        hipLaunchKernelGGL((batch_emit_kernel<M, LT, GT>), dim3((unsigned)groups), dim3(kThreads), 0, s, a);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, int64_t first_group, int64_t groups,

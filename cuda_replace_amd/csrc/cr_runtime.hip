@@ -24,12 +24,113 @@
 #include <vector>
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <string>
+#include <tuple>
 
 using namespace cr;
 
 namespace {
 
 constexpr size_t kAllocGranule = 2u << 20;   // device buffers grow in 2 MiB steps
+
+// ---- autotuner: tiles per group ------------------------------------------------
+// The group size (1, 2 or 4 tiles of 8 KiB per workgroup) trades the spine's work
+// and the per-workgroup fixed costs against LDS per workgroup (occupancy) and tail
+// balance.  The best one is measured, not assumed: the first pass of a pattern class
+// (border-free, carry tables of 2/4/8/16 states) in an input size bucket on a device
+// runs count + spine + emit once per candidate after its normal pass, times each
+// with HIP events (best of two), and records the fastest for every later pass in the
+// bucket.  Inputs under 4 MiB are not tuned.  CR_GROUP_TILES=1|2|4 fixes the choice;
+// CR_AUTOTUNE=0 keeps the default.
+constexpr int kTuneCands[3] = {1, 2, 4};
+
+struct TuneEntry {
+    int gt = 0;
+    float ms[3] = {0, 0, 0};   // per candidate, best of two
+};
+
+struct Tuner {
+    std::mutex mtx;
+    std::map<std::tuple<int, int, int>, TuneEntry> best;   // (device, class, bucket)
+};
+
+Tuner &tuner()
+{
+    static Tuner t;
+    return t;
+}
+
+int tune_class(int mode, int lt)
+{
+    if (mode == kModeBorderFree)
+        return 0;
+    if (mode == kModeTable)
+        return lt == 2 ? 1 : lt == 4 ? 2 : lt == 8 ? 3 : 4;
+    return -1;
+}
+
+// bucket 0 (under 4 MiB) is never measured, only pinned (cr_autotune_set)
+int size_bucket(int64_t n)
+{
+    if (n < (4ll << 20))
+        return 0;
+    if (n < (32ll << 20))
+        return 1;
+    if (n < (256ll << 20))
+        return 2;
+    if (n < (2048ll << 20))
+        return 3;
+    return 4;
+}
+
+int forced_group_tiles()
+{
+    static const int v = [] {
+        const char *e = getenv("CR_GROUP_TILES");
+        const int g = e ? atoi(e) : 0;
+        return g == 1 || g == 2 || g == 4 ? g : 0;
+    }();
+    return v;
+}
+
+bool autotune_enabled()
+{
+    static const bool v = [] {
+        const char *e = getenv("CR_AUTOTUNE");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
+// Group size for a pass; *tune is set when this pass should measure the candidates.
+int choose_group_tiles(int device, int mode, int lt, int64_t n, bool *tune)
+{
+    *tune = false;
+    if (mode == kModeBitmap)
+        return kGroupTiles;
+    if (const int f = forced_group_tiles())
+        return f;
+    const int c = tune_class(mode, lt), b = size_bucket(n);
+    if (c < 0)
+        return kGroupTiles;
+    Tuner &t = tuner();
+    std::lock_guard<std::mutex> lk(t.mtx);
+    auto it = t.best.find(std::make_tuple(device, c, b));
+    if (it != t.best.end() && it->second.gt > 0)
+        return it->second.gt;
+    *tune = b > 0 && autotune_enabled();
+    return kGroupTiles;
+}
+
+void record_tuning(int device, int mode, int lt, int64_t n, const TuneEntry &e)
+{
+    Tuner &t = tuner();
+    std::lock_guard<std::mutex> lk(t.mtx);
+    t.best[std::make_tuple(device, tune_class(mode, lt), size_bucket(n))] = e;
+}
+
+int64_t groups_of(int64_t num_tiles, int gt) { return (num_tiles + gt - 1) / gt; }
 
 struct Session {
     std::mutex mtx;
@@ -224,8 +325,9 @@ int64_t chunk_groups()
 hipError_t ensure_scratch(Session *s, int64_t num_tiles, int mode, int table_size)
 {
     hipError_t e;
-    // one word (border-free) or one table of table_size words per group
-    const int64_t groups = (num_tiles + kGroupTiles - 1) / kGroupTiles;
+    // one word (border-free) or one table of table_size words per group, sized for
+    // the smallest group (one tile) so every group size fits
+    const int64_t groups = num_tiles;
     const size_t per = mode == kModeTable ? (size_t)table_size : 1u;
     const size_t words = per * (size_t)groups;
     if (s->gsum_words < words) {
@@ -371,7 +473,9 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
 
     CK(upload_pattern(s, pat, L, rep, r));
     const int64_t num_tiles = (n + kTile - 1) / kTile;
-    const int64_t num_groups = (num_tiles + kGroupTiles - 1) / kGroupTiles;
+    bool tune = false;
+    const int gt = choose_group_tiles(s->device, mode, table_size, n, &tune);
+    const int64_t num_groups = groups_of(num_tiles, gt);
     const int64_t cg = chunk_groups();
     const int64_t num_chunks = (num_groups + cg - 1) / cg;
 
@@ -446,7 +550,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         p.dst = dst;
         p.out_len = exact;
         p.write_output = 1;
-        CK(launch_emit(p, kModeBitmap, 0, 0, num_groups, s->stream));
+        CK(launch_emit(p, kModeBitmap, 0, gt, 0, num_groups, s->stream));
         if (kernel_ms)
             CK(hipEventRecord(s->ev1, s->stream));
         CK(hipStreamSynchronize(s->stream));
@@ -459,14 +563,14 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             for (int64_t c = 0; c < num_chunks; ++c) {
                 const int64_t g0 = c * cg;
                 const int64_t gn = num_groups - g0 < cg ? num_groups - g0 : cg;
-                hipError_t e = launch_count(p, mode, table_size, g0, gn, s->stream);
+                hipError_t e = launch_count(p, mode, table_size, gt, g0, gn, s->stream);
                 if (e != hipSuccess)
                     return e;
                 e = launch_spine(p, mode, table_size, g0, gn, c, c == num_chunks - 1, s->stream);
                 if (e != hipSuccess)
                     return e;
                 if (emit) {
-                    e = launch_emit(p, mode, table_size, g0, gn, s->stream);
+                    e = launch_emit(p, mode, table_size, gt, g0, gn, s->stream);
                     if (e != hipSuccess)
                         return e;
                 }
@@ -508,7 +612,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         }
         p.dst = dst;
         if (counted) {
-            CK(launch_emit(p, mode, table_size, 0, num_groups, s->stream));
+            CK(launch_emit(p, mode, table_size, gt, 0, num_groups, s->stream));
         } else {
             if (kernel_ms)
                 CK(hipEventRecord(s->ev0, s->stream));
@@ -528,6 +632,34 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         }
         if (kernel_ms)
             CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
+        if (tune && num_chunks == 1) {
+            // every candidate recomputes the same summaries, states, records and
+            // output, so the pass's results above stay valid
+            TuneEntry te;
+            float best = 0.f;
+            for (int ci = 0; ci < 3; ++ci) {
+                const int cand = kTuneCands[ci];
+                const int64_t ng = groups_of(num_tiles, cand);
+                float tbest = 0.f;
+                for (int rep_i = 0; rep_i < 2; ++rep_i) {
+                    CK(hipEventRecord(s->ev0, s->stream));
+                    CK(launch_count(p, mode, table_size, cand, 0, ng, s->stream));
+                    CK(launch_spine(p, mode, table_size, 0, ng, 0, 1, s->stream));
+                    CK(launch_emit(p, mode, table_size, cand, 0, ng, s->stream));
+                    CK(hipEventRecord(s->ev1, s->stream));
+                    CK(hipEventSynchronize(s->ev1));
+                    float ms = 0.f;
+                    CK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                    tbest = rep_i == 0 || ms < tbest ? ms : tbest;
+                }
+                te.ms[ci] = tbest;
+                if (te.gt == 0 || tbest < best) {
+                    te.gt = cand;
+                    best = tbest;
+                }
+            }
+            record_tuning(s->device, mode, table_size, n, te);
+        }
     }
     if (mode_out)
         *mode_out = mode;
@@ -721,7 +853,7 @@ int session_apply_fused(Session *s)
     int64_t out = n;
     if (n > 0) {
         const int64_t num_tiles = (n + kTile - 1) / kTile;
-        const int64_t num_groups = (num_tiles + kGroupTiles - 1) / kGroupTiles;
+        const int64_t num_groups = groups_of(num_tiles, kGroupTiles);
         CK(ensure_scratch(s, num_tiles, kModeBorderFree, 0));
         CK(next_epoch(s));
         ReplaceParams p{};
@@ -1429,6 +1561,65 @@ int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d_dst, s
     if (out_last)
         *out_last = res.last;
     return 0;
+}
+
+int cr_autotune_set(const uint8_t *pat, int pat_len, size_t n, int group_tiles)
+{
+    if (pat_len < 0 || (pat_len > 0 && !pat))
+        return -1;
+    int lt = 0;
+    const int c = tune_class(pattern_mode(pat, pat_len, &lt), lt);
+    if (c < 0 || !(group_tiles == 0 || group_tiles == 1 || group_tiles == 2 || group_tiles == 4))
+        return -1;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return (int)e;
+    Tuner &t = tuner();
+    std::lock_guard<std::mutex> lk(t.mtx);
+    const auto key = std::make_tuple(dev, c, size_bucket((int64_t)n));
+    if (group_tiles == 0)
+        t.best.erase(key);
+    else
+        t.best[key] = TuneEntry{group_tiles, {0, 0, 0}};
+    return 0;
+}
+
+void cr_autotune_reset(void)
+{
+    Tuner &t = tuner();
+    std::lock_guard<std::mutex> lk(t.mtx);
+    t.best.clear();
+}
+
+int cr_autotune_report(char *buf, size_t cap)
+{
+    static const char *kClassName[5] = {"border_free", "table2", "table4", "table8", "table16"};
+    static const char *kBucket[5] = {"<4MiB", "4-32MiB", "32-256MiB", "256MiB-2GiB", ">=2GiB"};
+    std::string out = "[";
+    {
+        Tuner &t = tuner();
+        std::lock_guard<std::mutex> lk(t.mtx);
+        bool first = true;
+        for (const auto &kv : t.best) {
+            char line[256];
+            snprintf(line, sizeof line,
+                     "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"group_tiles\": %d, "
+                     "\"ms\": {\"1\": %.4f, \"2\": %.4f, \"4\": %.4f}}",
+                     first ? "" : ", ", std::get<0>(kv.first), kClassName[std::get<1>(kv.first)],
+                     kBucket[std::get<2>(kv.first)], kv.second.gt, kv.second.ms[0], kv.second.ms[1],
+                     kv.second.ms[2]);
+            out += line;
+            first = false;
+        }
+    }
+    out += "]";
+    if (buf && cap > 0) {
+        const size_t k = out.size() < cap - 1 ? out.size() : cap - 1;
+        memcpy(buf, out.data(), k);
+        buf[k] = 0;
+    }
+    return (int)out.size();
 }
 
 int cr_pattern_mode(const uint8_t *pat, int pat_len)

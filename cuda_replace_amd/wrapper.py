@@ -44,7 +44,7 @@ REQUIRED_SYMBOLS = (
     "cr_session_length", "cr_query_prefix64", "cr_replace_device", "cr_pattern_mode",
     "cr_batch_create", "cr_batch_run", "cr_batch_destroy", "cr_stream_replace",
     "cr_server_create", "cr_server_submit", "cr_server_wait", "cr_server_release", "cr_server_destroy",
-    "cr_apply_batch_ex", "cr_batch_fusable",
+    "cr_apply_batch_ex", "cr_batch_fusable", "cr_autotune_set", "cr_autotune_reset", "cr_autotune_report",
 )
 
 
@@ -115,6 +115,12 @@ class CudaReplaceLib:
         lib.cr_apply_batch_ex.restype = C.c_int
         lib.cr_batch_fusable.argtypes = [c_uint8_pp, c_int_p, c_uint8_pp, c_int_p, C.c_int]
         lib.cr_batch_fusable.restype = C.c_int
+        lib.cr_autotune_set.argtypes = [c_uint8_p, C.c_int, c_size_t, C.c_int]
+        lib.cr_autotune_set.restype = C.c_int
+        lib.cr_autotune_reset.argtypes = []
+        lib.cr_autotune_reset.restype = None
+        lib.cr_autotune_report.argtypes = [C.c_char_p, c_size_t]
+        lib.cr_autotune_report.restype = C.c_int
         lib.cuda_replace_result.argtypes = [c_void_p, C.POINTER(c_uint8_p), c_size_t_p]
         lib.cuda_replace_result.restype = C.c_int
         lib.cuda_replace_close.argtypes = [c_void_p]
@@ -169,6 +175,24 @@ class CudaReplaceLib:
 
     def pattern_mode(self, pat: bytes) -> int:
         return int(self.lib.cr_pattern_mode(_u8(pat), len(pat)))
+
+    def autotune_set(self, pat: bytes, n: int, group_tiles: int) -> None:
+        """Pins the tiles per group (1, 2 or 4; 0 unpins) for pat's pattern class and
+        the size bucket of n on the current device."""
+        rc = int(self.lib.cr_autotune_set(_u8(pat), len(pat), int(n), int(group_tiles)))
+        if rc != 0:
+            raise RuntimeError(f"cr_autotune_set failed rc={rc}")
+
+    def autotune_reset(self) -> None:
+        self.lib.cr_autotune_reset()
+
+    def autotune_report(self) -> list:
+        """The autotuner's current choices with their measured candidate times."""
+        import json
+        need = int(self.lib.cr_autotune_report(None, 0))
+        buf = C.create_string_buffer(need + 1)
+        self.lib.cr_autotune_report(buf, need + 1)
+        return json.loads(buf.value.decode())
 
     def batch_fusable(self, pairs: Sequence[Tuple[bytes, bytes]]) -> bool:
         """True if the library would run these pairs as one fused pass (host check

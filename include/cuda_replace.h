@@ -116,6 +116,24 @@ CR_API int cr_apply_batch_ex(void *h, const uint8_t *const *pats, const int *pat
 CR_API int cr_batch_fusable(const uint8_t *const *pats, const int *pat_lens,
                             const uint8_t *const *reps, const int *rep_lens, int count);
 
+/* Autotuner of the tiles per workgroup group (1, 2 or 4 tiles of 8 KiB).  The first
+ * pass of a pattern class (border-free, or carry tables of 2/4/8/16 states) in an
+ * input size bucket (under 4 MiB, 4-32 MiB, 32-256 MiB, 256 MiB-2 GiB, larger) on a
+ * device times each candidate after its normal pass and keeps the fastest;
+ * inputs under 4 MiB use 2 unless pinned.  CR_GROUP_TILES=1|2|4 in the environment
+ * fixes the choice and CR_AUTOTUNE=0 disables measuring.  No counterpart in the
+ * reference.
+ *
+ * cr_autotune_set pins group_tiles for the class of pattern pat and the bucket of
+ * n on the current device; 0 unpins.  Returns 0, or -1 for a pattern outside the
+ * tuned classes (empty, or the bitmap path) or a bad value. */
+CR_API int cr_autotune_set(const uint8_t *pat, int pat_len, size_t n, int group_tiles);
+/* Forgets every measured or pinned choice. */
+CR_API void cr_autotune_reset(void);
+/* JSON array of the current choices and their measured milliseconds per candidate,
+ * NUL-terminated into buf (truncated to cap); returns the full length. */
+CR_API int cr_autotune_report(char *buf, size_t cap);
+
 /* Library version string.  Replaces cuda_replace.cu:2251-2254. */
 CR_API const char *cuda_replace_version(void);
 
