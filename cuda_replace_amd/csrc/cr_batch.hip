@@ -21,8 +21,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <mutex>
 #include <new>
+#include <stdlib.h>
 #include <string.h>
 #include <thread>
 #include <vector>
@@ -151,6 +153,7 @@ struct DocState {
 struct Engine {
     std::mutex mtx;
     int device = 0;
+    int num_cus = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf arena[2], fin;
@@ -159,6 +162,16 @@ struct Engine {
     // round j's uploads may still read the other (round j+1's sync orders them)
     HostBuf h_docs[2], h_groups[2], h_cdocs[2], h_pool[2];
     HostBuf h_in, h_kept, h_out, h_misc;
+    // device-driven path
+    int mode = 0;   // 0 automatic, 1 host-driven rounds only, 2 device-driven when the pairs allow
+    DevBuf g_pairs, g_pool, g_states, g_states0, g_ctl, g_ctl0, g_docs, g_kept;
+    DevBuf g_groups[kBatchClasses], g_gsum[kBatchClasses], g_pre[kBatchClasses], g_cdocs[kBatchClasses + 1];
+    HostBuf h_pairs, h_states, h_ctl;
+    size_t hint_arena = 0, hint_fin = 0;       // sizes learnt from earlier runs
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<int64_t> graph_key;
+    double info[6] = {0, 0, 0, 0, 0, 0};       // last run: path, tiles per group, graph reused, tuned, rounds
 };
 
 struct DeviceGuard {
@@ -587,16 +600,401 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
     return 0;
 }
 
+// ---- device-driven runs -------------------------------------------------------
+// Every round of the run is planned, counted, placed and emitted on the GPU
+// (launch_batch_round), so the host issues the whole run at once: captured as a HIP
+// graph, replayed while the run's shape and the engine's buffers stay the same.
+// The work lists and arenas are sized from the input (and from earlier runs); a run
+// that would not fit sets the device's overflow flag, the host-driven rounds redo
+// it, and the next run sizes up.
+constexpr int kFallback = 1 << 20;   // internal: redo the run with host-driven rounds
+
+int env_int(const char *name, int dflt)
+{
+    const char *e = getenv(name);
+    return e && *e ? atoi(e) : dflt;
+}
+
+// Tiles per group for device-driven runs, per device and input size bucket: the
+// first run of a bucket executes round 0 once per candidate (1, 2, 4) from the same
+// starting state, timed with HIP events, and keeps the fastest.
+std::mutex g_tune_mtx;
+std::map<std::pair<int, int>, std::pair<int, std::vector<float>>> g_tune;
+
+int batch_bucket(size_t bytes)
+{
+    if (bytes < (4u << 20))
+        return 0;
+    if (bytes < (64u << 20))
+        return 1;
+    if (bytes < ((size_t)1 << 30))
+        return 2;
+    return 3;
+}
+
+BatchRoundShape round_shape(const std::vector<DocState> &st, const std::vector<int8_t> &pcls, int j, int gt,
+                            int grid_max)
+{
+    BatchRoundShape sh{};
+    int64_t est[kBatchClasses] = {0, 0, 0, 0, 0};
+    for (const DocState &d : st) {
+        if (d.npairs <= j)
+            continue;
+        const int c = pcls[(size_t)(d.pair0 + j)];
+        sh.present[c] = 1;
+        sh.class_docs[c] += 1;
+        if (c < kBatchClasses)
+            est[c] += ((d.len + kTile - 1) / kTile + gt - 1) / gt + 1;
+    }
+    for (int c = 0; c < kBatchClasses; ++c)
+        sh.work_grid[c] = (int)std::max<int64_t>(1, std::min<int64_t>(est[c], grid_max));
+    return sh;
+}
+
+bool valid_run(int num_docs, const uint8_t *const *docs, const size_t *doc_lens, const int *pair_counts,
+               const uint8_t *const *pats, const int *pat_lens, const uint8_t *const *reps, const int *rep_lens,
+               const uint8_t **out_base, size_t *out_offsets, size_t *out_lens)
+{
+    if (num_docs < 0 || !out_base || (num_docs > 0 && (!docs || !doc_lens || !pair_counts || !out_offsets ||
+                                                         !out_lens)))
+        return false;
+    int64_t total_pairs = 0;
+    for (int i = 0; i < num_docs; ++i) {
+        if (pair_counts[i] < 0 || (doc_lens[i] > 0 && !docs[i]))
+            return false;
+        total_pairs += pair_counts[i];
+    }
+    if (total_pairs > 0 && (!pats || !pat_lens || !reps || !rep_lens))
+        return false;
+    for (int64_t q = 0; q < total_pairs; ++q)
+        if (pat_lens[q] < 0 || rep_lens[q] < 0 || (pat_lens[q] > 0 && !pats[q]) || (rep_lens[q] > 0 && !reps[q]))
+            return false;
+    return true;
+}
+
+int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const size_t *doc_lens,
+                      const int *pair_counts, const uint8_t *const *pats, const int *pat_lens,
+                      const uint8_t *const *reps, const int *rep_lens, const uint8_t **out_base,
+                      size_t *out_offsets, size_t *out_lens, double *stats)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    hipStream_t s = E->stream;
+    std::vector<DocState> st((size_t)num_docs);
+    int64_t total_pairs = 0;
+    int max_pairs = 0;
+    size_t total_in = 0, arena_in = 0, fin0 = 0;
+    std::vector<Piece> pieces;
+    pieces.reserve((size_t)num_docs);
+    for (int i = 0; i < num_docs; ++i) {
+        DocState &d = st[(size_t)i];
+        d.pair0 = total_pairs;
+        d.npairs = pair_counts[i];
+        d.len = (int64_t)doc_lens[i];
+        total_pairs += pair_counts[i];
+        max_pairs = std::max(max_pairs, pair_counts[i]);
+        pieces.push_back(Piece{arena_in, docs[i], doc_lens[i]});
+        total_in += doc_lens[i];
+        arena_in += align_up(doc_lens[i], kDocAlign);
+        if (d.npairs == 0) {
+            d.fin_off = fin0;
+            fin0 += align_up(doc_lens[i], kDocAlign);
+        }
+    }
+    // pair classes; the bitmap path is host-driven only
+    std::vector<int8_t> pcls((size_t)total_pairs);
+    size_t pool = 0;
+    for (int64_t q = 0; q < total_pairs; ++q) {
+        int lt = 0;
+        const int mode = pattern_mode(pats[q], pat_lens[q], &lt);
+        if (mode == kModeEmpty) {
+            pcls[(size_t)q] = kBatchEmpty;
+        } else {
+            const int c = class_of(mode, lt);
+            if (c >= kClasses)
+                return kFallback;
+            pcls[(size_t)q] = (int8_t)c;
+        }
+        pool += align_up((size_t)pat_lens[q] + 16, 16) + align_up((size_t)rep_lens[q] + 32, 16);
+    }
+
+    // ---- buffers ------------------------------------------------------------------
+    const size_t grow_hint = arena_in + arena_in / 4 + (1u << 20);
+    const size_t cap_a = std::max(grow_hint, E->hint_arena);
+    CK(dev_reserve(E->arena[0], cap_a, false, 0, s));
+    CK(dev_reserve(E->arena[1], cap_a, false, 0, s));
+    CK(dev_reserve(E->fin, fin0 + std::max(grow_hint, E->hint_fin), false, 0, s));
+    const int64_t group_cap = (int64_t)(std::max(E->arena[0].cap, E->arena[1].cap) / kTile) + num_docs + 1;
+    static const int kWords[kBatchClasses] = {1, 2, 4, 8, 16};
+    for (int c = 0; c < kBatchClasses; ++c) {
+        CK(dev_reserve(E->g_groups[c], (size_t)group_cap * sizeof(BatchGroup), false, 0, s));
+        CK(dev_reserve(E->g_gsum[c], (size_t)group_cap * kWords[c] * sizeof(uint32_t), false, 0, s));
+        CK(dev_reserve(E->g_pre[c], (size_t)group_cap * sizeof(ChunkState), false, 0, s));
+    }
+    const size_t D = (size_t)std::max(num_docs, 1);
+    for (int c = 0; c <= kBatchClasses; ++c)
+        CK(dev_reserve(E->g_cdocs[c], D * sizeof(int32_t), false, 0, s));
+    CK(dev_reserve(E->g_docs, D * sizeof(BatchDoc), false, 0, s));
+    CK(dev_reserve(E->g_kept, D * sizeof(int64_t), false, 0, s));
+    CK(dev_reserve(E->g_states, D * sizeof(BatchDocState), false, 0, s));
+    CK(dev_reserve(E->g_states0, D * sizeof(BatchDocState), false, 0, s));
+    CK(dev_reserve(E->g_ctl, sizeof(BatchCtl), false, 0, s));
+    CK(dev_reserve(E->g_ctl0, sizeof(BatchCtl), false, 0, s));
+    CK(dev_reserve(E->g_pairs, (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(BatchPair), false, 0, s));
+    CK(dev_reserve(E->g_pool, pool + 64, false, 0, s));
+
+    // ---- staging and uploads --------------------------------------------------------
+    CK(host_reserve(E->h_in, arena_in + 1));
+    CK(host_reserve(E->h_pairs, (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(BatchPair) + pool + 64));
+    CK(host_reserve(E->h_states, D * sizeof(BatchDocState) + sizeof(BatchCtl) + 64));
+    CK(host_reserve(E->h_ctl, sizeof(BatchCtl) + 64));
+    copy_pieces(E->h_in.p, pieces, total_in);
+    BatchPair *hp = reinterpret_cast<BatchPair *>(E->h_pairs.p);
+    uint8_t *hpool = E->h_pairs.p + (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(BatchPair);
+    size_t pool_fill = 0;
+    for (int64_t q = 0; q < total_pairs; ++q) {
+        BatchPair bp{};
+        bp.L = pat_lens[q];
+        bp.r = rep_lens[q];
+        pattern_head(pats[q], bp.L, &bp.pat4, &bp.mask4);
+        bp.cls = pcls[(size_t)q];
+        bp.pat_off = (int32_t)pool_fill;
+        if (bp.L > 0)
+            memcpy(hpool + pool_fill, pats[q], (size_t)bp.L);
+        pool_fill += align_up((size_t)bp.L + 16, 16);
+        bp.rep_off = (int32_t)pool_fill;
+        if (bp.r > 0)
+            memcpy(hpool + pool_fill, reps[q], (size_t)bp.r);
+        pool_fill += align_up((size_t)bp.r + 32, 16);
+        hp[q] = bp;
+    }
+    BatchDocState *hs = reinterpret_cast<BatchDocState *>(E->h_states.p);
+    for (int i = 0; i < num_docs; ++i) {
+        const DocState &d = st[(size_t)i];
+        hs[i] = BatchDocState{d.len, (int64_t)pieces[(size_t)i].dst_off, (int64_t)d.fin_off, (int32_t)d.pair0,
+                              d.npairs};
+    }
+    BatchCtl *hc = reinterpret_cast<BatchCtl *>(E->h_ctl.p);
+    memset(hc, 0, sizeof(BatchCtl));
+    hc->fin_used = (int64_t)fin0;
+    if (arena_in > 0)
+        CK(hipMemcpyAsync(E->arena[0].p, E->h_in.p, arena_in, hipMemcpyHostToDevice, s));
+    if (total_pairs > 0)
+        CK(hipMemcpyAsync(E->g_pairs.p, hp, (size_t)total_pairs * sizeof(BatchPair), hipMemcpyHostToDevice, s));
+    if (pool_fill > 0)
+        CK(hipMemcpyAsync(E->g_pool.p, hpool, pool_fill, hipMemcpyHostToDevice, s));
+    if (num_docs > 0)
+        CK(hipMemcpyAsync(E->g_states.p, hs, (size_t)num_docs * sizeof(BatchDocState), hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(E->g_ctl.p, hc, sizeof(BatchCtl), hipMemcpyHostToDevice, s));
+    for (int i = 0; i < num_docs; ++i) {
+        const DocState &d = st[(size_t)i];
+        if (d.npairs == 0 && d.len > 0)
+            CK(hipMemcpyAsync(E->fin.p + d.fin_off, E->arena[0].p + pieces[(size_t)i].dst_off, (size_t)d.len,
+                              hipMemcpyDeviceToDevice, s));
+    }
+
+    // ---- rounds -------------------------------------------------------------------
+    BatchRound base{};
+    base.ctl = reinterpret_cast<BatchCtl *>(E->g_ctl.p);
+    base.states = reinterpret_cast<BatchDocState *>(E->g_states.p);
+    base.pairs = reinterpret_cast<const BatchPair *>(E->g_pairs.p);
+    base.pool = E->g_pool.p;
+    base.docs = reinterpret_cast<BatchDoc *>(E->g_docs.p);
+    for (int c = 0; c < kBatchClasses; ++c) {
+        base.groups[c] = reinterpret_cast<BatchGroup *>(E->g_groups[c].p);
+        base.gsum[c] = reinterpret_cast<uint32_t *>(E->g_gsum[c].p);
+        base.pre[c] = reinterpret_cast<ChunkState *>(E->g_pre[c].p);
+    }
+    for (int c = 0; c <= kBatchClasses; ++c)
+        base.cdocs[c] = reinterpret_cast<int32_t *>(E->g_cdocs[c].p);
+    base.doc_kept = reinterpret_cast<int64_t *>(E->g_kept.p);
+    base.fin = E->fin.p;
+    base.fin_cap = (int64_t)E->fin.cap;
+    base.group_cap = group_cap;
+    base.num_docs = num_docs;
+    auto round_args = [&](int j, int gt) {
+        BatchRound a = base;
+        a.round = j;
+        a.gt = gt;
+        a.src_arena = E->arena[j & 1].p;
+        a.dst_arena = E->arena[(j + 1) & 1].p;
+        a.dst_cap = (int64_t)E->arena[(j + 1) & 1].cap;
+        return a;
+    };
+    const int grid_max = 8 * E->num_cus;
+
+    int gt = env_int("CR_GROUP_TILES", 0);
+    if (gt != 1 && gt != 2 && gt != 4)
+        gt = 0;
+    bool tuned_now = false;
+    const auto tkey = std::make_pair(E->device, batch_bucket(arena_in));
+    if (gt == 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        auto it = g_tune.find(tkey);
+        if (it != g_tune.end())
+            gt = it->second.first;
+    }
+    if (gt == 0 && (env_int("CR_AUTOTUNE", 1) == 0 || tkey.second == 0 || max_pairs == 0))
+        gt = kGroupTiles;
+    if (gt == 0) {
+        // round 0 once per candidate, each from the uploaded state
+        const size_t sbytes = (size_t)num_docs * sizeof(BatchDocState);
+        if (sbytes > 0)
+            CK(hipMemcpyAsync(E->g_states0.p, E->g_states.p, sbytes, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(E->g_ctl0.p, E->g_ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));
+        std::vector<float> ms(3, 0.f);
+        float best = 0.f;
+        static const int kCands[3] = {1, 2, 4};
+        for (int ci = 0; ci < 3; ++ci) {
+            if (sbytes > 0)
+                CK(hipMemcpyAsync(E->g_states.p, E->g_states0.p, sbytes, hipMemcpyDeviceToDevice, s));
+            CK(hipMemcpyAsync(E->g_ctl.p, E->g_ctl0.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));
+            CK(hipEventRecord(E->ev0, s));
+            CK(launch_batch_round(round_args(0, kCands[ci]), round_shape(st, pcls, 0, kCands[ci], grid_max), s));
+            CK(hipEventRecord(E->ev1, s));
+            CK(hipEventSynchronize(E->ev1));
+            CK(hipEventElapsedTime(&ms[(size_t)ci], E->ev0, E->ev1));
+            if (gt == 0 || ms[(size_t)ci] < best) {
+                gt = kCands[ci];
+                best = ms[(size_t)ci];
+            }
+        }
+        if (sbytes > 0)
+            CK(hipMemcpyAsync(E->g_states.p, E->g_states0.p, sbytes, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(E->g_ctl.p, E->g_ctl0.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        g_tune[tkey] = std::make_pair(gt, ms);
+        tuned_now = true;
+    }
+
+    std::vector<BatchRoundShape> shapes((size_t)max_pairs);
+    std::vector<int64_t> key = {num_docs, max_pairs, gt, group_cap, (int64_t)E->fin.cap, (int64_t)E->arena[0].cap,
+                                (int64_t)E->arena[1].cap};
+    for (int j = 0; j < max_pairs; ++j) {
+        shapes[(size_t)j] = round_shape(st, pcls, j, gt, grid_max);
+        const BatchRoundShape &sh = shapes[(size_t)j];
+        for (int c = 0; c <= kBatchClasses; ++c)
+            key.push_back(sh.present[c] ? sh.class_docs[c] : 0);
+        for (int c = 0; c < kBatchClasses; ++c)
+            key.push_back(sh.work_grid[c]);
+    }
+    const void *ptrs[] = {E->arena[0].p, E->arena[1].p, E->fin.p, E->g_pairs.p, E->g_pool.p, E->g_states.p,
+                          E->g_ctl.p, E->g_docs.p, E->g_kept.p};
+    for (const void *q : ptrs)
+        key.push_back((int64_t)(uintptr_t)q);
+    for (int c = 0; c < kBatchClasses; ++c) {
+        key.push_back((int64_t)(uintptr_t)E->g_groups[c].p);
+        key.push_back((int64_t)(uintptr_t)E->g_gsum[c].p);
+        key.push_back((int64_t)(uintptr_t)E->g_pre[c].p);
+    }
+    for (int c = 0; c <= kBatchClasses; ++c)
+        key.push_back((int64_t)(uintptr_t)E->g_cdocs[c].p);
+
+    auto launch_rounds = [&]() -> hipError_t {
+        for (int j = 0; j < max_pairs; ++j) {
+            hipError_t e = launch_batch_round(round_args(j, gt), shapes[(size_t)j], s);
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
+    };
+    bool reused = false;
+    const bool use_graph = env_int("CR_BATCH_GRAPH", 1) != 0 && max_pairs > 0;
+    if (use_graph && !(E->graph_exec && key == E->graph_key)) {
+        if (E->graph_exec)
+            (void)hipGraphExecDestroy(E->graph_exec);
+        if (E->graph)
+            (void)hipGraphDestroy(E->graph);
+        E->graph_exec = nullptr;
+        E->graph = nullptr;
+        E->graph_key.clear();
+        CK(hipStreamSynchronize(s));
+        CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = launch_rounds();
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(s, &g);
+        if (le != hipSuccess || ee != hipSuccess) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return (int)(le != hipSuccess ? le : ee);
+        }
+        hipGraphExec_t x = nullptr;
+        hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return (int)ie;
+        }
+        E->graph = g;
+        E->graph_exec = x;
+        E->graph_key = key;
+    } else if (use_graph) {
+        reused = true;
+    }
+    CK(hipEventRecord(E->ev0, s));
+    if (use_graph)
+        CK(hipGraphLaunch(E->graph_exec, s));
+    else
+        CK(launch_rounds());
+    CK(hipEventRecord(E->ev1, s));
+
+    // ---- outcome ----------------------------------------------------------------------
+    CK(hipMemcpyAsync(hc, E->g_ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
+    if (num_docs > 0)
+        CK(hipMemcpyAsync(hs, E->g_states.p, (size_t)num_docs * sizeof(BatchDocState), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    if (hc->overflow) {
+        E->hint_arena = 2 * std::max(E->arena[0].cap, E->arena[1].cap);
+        E->hint_fin = 2 * (E->fin.cap - fin0);
+        return kFallback;
+    }
+    const size_t fin_used = (size_t)hc->fin_used;
+    CK(host_reserve(E->h_out, fin_used + 64));
+    if (fin_used > 0)
+        CK(hipMemcpyAsync(E->h_out.p, E->fin.p, fin_used, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    for (int i = 0; i < num_docs; ++i) {
+        out_offsets[i] = (size_t)hs[i].fin_off;
+        out_lens[i] = (size_t)hs[i].len;
+    }
+    *out_base = E->h_out.p;
+    if (stats) {
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+        stats[0] = ms;
+        stats[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        stats[2] = (double)hc->algo_bytes;
+        stats[3] = max_pairs;
+    }
+    E->info[0] = 2;
+    E->info[1] = gt;
+    E->info[2] = reused ? 1 : 0;
+    E->info[3] = tuned_now ? 1 : 0;
+    E->info[4] = max_pairs;
+    E->info[5] = use_graph ? 1 : 0;
+    return 0;
+}
+
 void engine_free(Engine *E)
 {
+    if (E->graph_exec)
+        (void)hipGraphExecDestroy(E->graph_exec);
+    if (E->graph)
+        (void)hipGraphDestroy(E->graph);
     DevBuf *dev[] = {&E->arena[0], &E->arena[1], &E->fin, &E->d_docs, &E->d_groups, &E->d_cdocs,
-                     &E->d_pool, &E->d_gsum, &E->d_pre, &E->d_kept, &E->d_other};
+                     &E->d_pool, &E->d_gsum, &E->d_pre, &E->d_kept, &E->d_other, &E->g_pairs, &E->g_pool,
+                     &E->g_states, &E->g_states0, &E->g_ctl, &E->g_ctl0, &E->g_docs, &E->g_kept};
     for (DevBuf *b : dev)
         if (b->p)
             (void)hipFree(b->p);
+    for (int c = 0; c <= kBatchClasses; ++c) {
+        DevBuf *per[] = {c < kBatchClasses ? &E->g_groups[c] : nullptr, c < kBatchClasses ? &E->g_gsum[c] : nullptr,
+                         c < kBatchClasses ? &E->g_pre[c] : nullptr, &E->g_cdocs[c]};
+        for (DevBuf *b : per)
+            if (b && b->p)
+                (void)hipFree(b->p);
+    }
     HostBuf *host[] = {&E->h_in, &E->h_docs[0], &E->h_docs[1], &E->h_groups[0], &E->h_groups[1],
                        &E->h_cdocs[0], &E->h_cdocs[1], &E->h_pool[0], &E->h_pool[1], &E->h_kept,
-                       &E->h_out, &E->h_misc};
+                       &E->h_out, &E->h_misc, &E->h_pairs, &E->h_states, &E->h_ctl};
     for (HostBuf *b : host)
         if (b->p)
             (void)hipHostFree(b->p);
@@ -627,11 +1025,18 @@ CR_API int cr_batch_create(void **pb)
         e = hipEventCreate(&E->ev0);
     if (e == hipSuccess)
         e = hipEventCreate(&E->ev1);
+    if (e == hipSuccess) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, E->device) == hipSuccess && cus > 0)
+            E->num_cus = cus;
+    }
     if (e != hipSuccess) {
         engine_free(E);
         delete E;
         return (int)e;
     }
+    const int m = env_int("CR_BATCH_DEVICE", 1);
+    E->mode = m == 0 ? 1 : 0;
     *pb = E;
     return 0;
 }
@@ -646,8 +1051,48 @@ CR_API int cr_batch_run(void *b, int num_docs, const uint8_t *const *docs, const
     Engine *E = static_cast<Engine *>(b);
     std::lock_guard<std::mutex> lock(E->mtx);
     DeviceGuard dg(E->device);
-    return engine_run(E, num_docs, docs, doc_lens, pair_counts, pats, pat_lens, reps, rep_lens, out_base,
-                      out_offsets, out_lens, stats);
+    if (E->mode != 1 && valid_run(num_docs, docs, doc_lens, pair_counts, pats, pat_lens, reps, rep_lens, out_base,
+                                  out_offsets, out_lens)) {
+        const int rc = engine_run_device(E, num_docs, docs, doc_lens, pair_counts, pats, pat_lens, reps, rep_lens,
+                                         out_base, out_offsets, out_lens, stats);
+        if (rc != kFallback)
+            return rc;
+    }
+    const int rc = engine_run(E, num_docs, docs, doc_lens, pair_counts, pats, pat_lens, reps, rep_lens, out_base,
+                              out_offsets, out_lens, stats);
+    if (rc == 0) {
+        E->info[0] = 1;
+        E->info[1] = kGroupTiles;
+        E->info[2] = E->info[3] = E->info[5] = 0;
+        E->info[4] = stats ? stats[3] : 0;
+    }
+    return rc;
+}
+
+// mode: 0 automatic (device-driven rounds whenever no pair needs the bitmap path),
+// 1 host-driven rounds only.
+CR_API int cr_batch_set_mode(void *b, int mode)
+{
+    if (!b || (mode != 0 && mode != 1))
+        return -1;
+    Engine *E = static_cast<Engine *>(b);
+    std::lock_guard<std::mutex> lock(E->mtx);
+    E->mode = mode;
+    return 0;
+}
+
+// The last run: info[0] path (1 host-driven rounds, 2 device-driven), [1] tiles per
+// group, [2] HIP graph replayed without capture, [3] group size measured in this
+// run, [4] rounds, [5] run as a HIP graph.  n: entries wanted (up to 6).
+CR_API int cr_batch_info(void *b, double *info, int n)
+{
+    if (!b || !info || n < 0)
+        return -1;
+    Engine *E = static_cast<Engine *>(b);
+    std::lock_guard<std::mutex> lock(E->mtx);
+    for (int i = 0; i < n && i < 6; ++i)
+        info[i] = E->info[i];
+    return 0;
 }
 
 CR_API void cr_batch_destroy(void *b)

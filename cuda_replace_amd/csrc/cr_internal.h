@@ -175,6 +175,79 @@ hipError_t launch_batch_count(const BatchArgs &a, int mode, int table_size, int 
 hipError_t launch_batch_spine(const BatchArgs &a, int mode, int table_size, int64_t docs, hipStream_t s);
 hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int gt, int64_t groups, hipStream_t s);
 
+// ---- device-driven batch: every round planned and placed on the GPU ----------------
+// Lengths never come back to the host between rounds, so a whole run is a fixed
+// launch sequence (captured once as a HIP graph and replayed).  Per round: plan
+// (one workgroup: each active document's class, group count and group range),
+// expand (the class work lists), per class a persistent count over a work queue and
+// a persistent spine, place (one workgroup: new lengths and destinations, packed in
+// the ping-pong arena or the final arena), per class a persistent emit, and the
+// empty-pattern documents.  A plan or placement that would not fit the buffers sets
+// the overflow flag and every later kernel of the run returns at once.
+constexpr int kBatchClasses = 5;       // border-free, carry tables of 2/4/8/16 states
+constexpr int kBatchEmpty = 5;         // class index of empty-pattern pairs
+
+struct BatchPair {
+    int32_t L;
+    int32_t r;
+    uint32_t pat4;
+    uint32_t mask4;
+    int32_t pat_off;       // into the run's pool
+    int32_t rep_off;
+    int32_t cls;           // 0..4 counted classes, kBatchEmpty
+    int32_t pad;
+};
+
+struct BatchDocState {
+    int64_t len;           // current length
+    int64_t off;           // offset of the current bytes in the round's source arena
+    int64_t fin_off;       // offset in the final arena once the last pair ran
+    int32_t pair0;         // first pair of the document
+    int32_t npairs;
+};
+
+struct BatchCtl {
+    int64_t class_groups[kBatchClasses];
+    int32_t class_docs[kBatchClasses + 1];
+    uint32_t queue[2][kBatchClasses];   // work-queue heads: count, emit
+    int64_t fin_used;                   // final arena bytes handed out so far
+    int64_t algo_bytes;                 // input + output bytes of every applied pair
+    int32_t overflow;
+    int32_t pad;
+};
+
+struct BatchRound {
+    BatchCtl *ctl;
+    BatchDocState *states;
+    const BatchPair *pairs;
+    const uint8_t *pool;
+    BatchDoc *docs;                         // indexed by document
+    BatchGroup *groups[kBatchClasses];      // work lists, group_cap entries each
+    int32_t *cdocs[kBatchClasses + 1];      // documents per class, num_docs entries each
+    uint32_t *gsum[kBatchClasses];          // group summaries (group_cap x words)
+    ChunkState *pre[kBatchClasses];         // group states (group_cap)
+    int64_t *doc_kept;
+    const uint8_t *src_arena;
+    uint8_t *dst_arena;
+    uint8_t *fin;
+    int64_t dst_cap;
+    int64_t fin_cap;
+    int64_t group_cap;
+    int32_t num_docs;
+    int32_t round;
+    int32_t gt;                             // tiles per group
+    int32_t pad;
+};
+
+struct BatchRoundShape {
+    int present[kBatchClasses + 1];         // class has documents this round
+    int class_docs[kBatchClasses + 1];      // exact, known from the pairs
+    int work_grid[kBatchClasses];           // persistent count / emit workgroups
+};
+
+// One whole round of a device-driven batch on stream s (capturable).
+hipError_t launch_batch_round(const BatchRound &a, const BatchRoundShape &shape, hipStream_t s);
+
 hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, int64_t first_group,
                         int64_t groups, hipStream_t s);   // which: 0 count, 1 spine, 2 emit
 

@@ -1056,19 +1056,18 @@ batch_emit_kernel(BatchArgs a)
 // groups per step, starting from no kept start and carry 0.  Writes every group's
 // exclusive state and the document's kept total.
 template <int MODE, int LT>
-__global__ void __launch_bounds__(kWave)
-batch_spine_kernel(BatchArgs a)
+__device__ __forceinline__ void batch_spine_doc(const BatchDoc *docs, const uint32_t *gsum, ChunkState *pre,
+                                                int64_t *doc_kept, int doc)
 {
     const int lane = threadIdx.x;
-    const int doc = a.class_docs[blockIdx.x];
-    const BatchDoc d = a.docs[doc];
+    const BatchDoc d = docs[doc];
     int64_t k = 0;
     uint32_t carry = 0;
     for (int64_t b = 0; b < d.ngroups; b += kWave) {
         const int64_t gi = b + lane;
         const int64_t slot = d.group0 + gi;
         if constexpr (MODE != kModeTable) {
-            const uint32_t v = gi < d.ngroups ? a.gsum[slot] : 0u;
+            const uint32_t v = gi < d.ngroups ? gsum[slot] : 0u;
             uint32_t inc = v;
 #pragma unroll
             for (int dd = 1; dd < kWave; dd <<= 1) {
@@ -1081,7 +1080,7 @@ batch_spine_kernel(BatchArgs a)
                 cs.k_base = k + inc - v;
                 cs.carry = 0;
                 cs.pad = 0;
-                a.pre[slot] = cs;
+                pre[slot] = cs;
             }
             k += __shfl(inc, kWave - 1, kWave);
         } else {
@@ -1089,7 +1088,7 @@ batch_spine_kernel(BatchArgs a)
             if (gi < d.ngroups) {
 #pragma unroll
                 for (int c = 0; c < LT; ++c)
-                    t.e[c] = a.gsum[slot * LT + c];
+                    t.e[c] = gsum[slot * LT + c];
             }
             Table64<LT> inc = widen<LT>(t);
 #pragma unroll
@@ -1113,7 +1112,7 @@ batch_spine_kernel(BatchArgs a)
                 cs.k_base = k + (int64_t)(e & kCountMask);
                 cs.carry = (int)(e >> kCarryShift);
                 cs.pad = 0;
-                a.pre[slot] = cs;
+                pre[slot] = cs;
             }
             const uint64_t el = tsel64<LT>(wave_broadcast_last<LT>(inc), carry);
             k += (int64_t)(el & kCountMask);
@@ -1121,7 +1120,284 @@ batch_spine_kernel(BatchArgs a)
         }
     }
     if (lane == 0)
-        a.doc_kept[doc] = k;
+        doc_kept[doc] = k;
+}
+
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kWave)
+batch_spine_kernel(BatchArgs a)
+{
+    batch_spine_doc<MODE, LT>(a.docs, a.gsum, a.pre, a.doc_kept, a.class_docs[blockIdx.x]);
+}
+
+// ---- device-driven batch rounds ----------------------------------------------
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanWaves = kPlanThreads / kWave;
+
+__device__ __forceinline__ ReplaceParams doc_view(const BatchDoc &d, const uint8_t *pool)
+{
+    ReplaceParams p{};
+    p.src = d.src;
+    p.dst = d.dst;
+    p.n = d.n;
+    p.floor = 0;
+    p.out_len = -1;
+    p.pat = pool + d.pat_off;
+    p.rep = pool + d.rep_off;
+    p.L = d.L;
+    p.r = d.r;
+    p.pat4 = d.pat4;
+    p.mask4 = d.mask4;
+    p.num_tiles = (d.n + kTile - 1) / kTile;
+    p.num_groups = d.ngroups;
+    p.write_output = 1;
+    return p;
+}
+
+__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int64_t u = __shfl_up(v, d, kWave);
+        if (lane >= d)
+            v += u;
+    }
+    return v;
+}
+
+// Exclusive scan over the workgroup (kPlanThreads) of v; *total gets the sum.
+// s_w: kPlanWaves scratch slots.  Contains barriers: call from every thread.
+__device__ __forceinline__ int64_t block_excl_scan_i64(int64_t v, int64_t *s_w, int64_t *total)
+{
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int64_t inc = wave_incl_scan_i64(v, lane);
+    if (lane == kWave - 1)
+        s_w[wave] = inc;
+    __syncthreads();
+    int64_t before = 0, sum = 0;
+    for (int w = 0; w < kPlanWaves; ++w) {
+        const int64_t x = s_w[w];
+        before += w < wave ? x : 0;
+        sum += x;
+    }
+    __syncthreads();   // s_w is reused by the next scan
+    *total = sum;
+    return before + inc - v;
+}
+
+// Round plan: every active document's class, descriptor and group range in its
+// class's work list, and the class document lists; resets the work queues.
+__global__ void __launch_bounds__(kPlanThreads) batch_plan_kernel(BatchRound a)
+{
+    __shared__ int64_t s_w[kPlanWaves];
+    const int tid = threadIdx.x;
+    BatchCtl *ctl = a.ctl;
+    int64_t gbase[kBatchClasses] = {0, 0, 0, 0, 0};
+    int64_t dbase[kBatchClasses + 1] = {0, 0, 0, 0, 0, 0};
+    bool over = false;
+    for (int base = 0; base < a.num_docs; base += kPlanThreads) {
+        const int i = base + tid;
+        int cls = -1;
+        int64_t ng = 0;
+        BatchDocState st{};
+        BatchPair pr{};
+        if (i < a.num_docs) {
+            st = a.states[i];
+            if (st.npairs > a.round) {
+                pr = a.pairs[st.pair0 + a.round];
+                cls = pr.cls;
+                if (cls < kBatchClasses)
+                    ng = ((st.len + kTile - 1) / kTile + a.gt - 1) / a.gt;
+            }
+        }
+        int64_t g0 = 0, d0 = 0;
+        for (int c = 0; c <= kBatchClasses; ++c) {
+            int64_t tot;
+            if (c < kBatchClasses) {
+                const int64_t ex = block_excl_scan_i64(cls == c ? ng : 0, s_w, &tot);
+                if (cls == c)
+                    g0 = gbase[c] + ex;
+                gbase[c] += tot;
+            }
+            const int64_t exd = block_excl_scan_i64(cls == c ? 1 : 0, s_w, &tot);
+            if (cls == c)
+                d0 = dbase[c] + exd;
+            dbase[c] += tot;
+        }
+        if (cls >= 0) {
+            BatchDoc bd{};
+            bd.src = a.src_arena + st.off;
+            bd.dst = nullptr;
+            bd.n = st.len;
+            bd.group0 = g0;
+            bd.ngroups = (int32_t)ng;
+            bd.L = pr.L;
+            bd.r = pr.r;
+            bd.pat4 = pr.pat4;
+            bd.mask4 = pr.mask4;
+            bd.pat_off = pr.pat_off;
+            bd.rep_off = pr.rep_off;
+            a.docs[i] = bd;
+            a.cdocs[cls][d0] = i;
+            if (cls < kBatchClasses && g0 + ng > a.group_cap)
+                over = true;
+        }
+    }
+    if (over)
+        ctl->overflow = 1;
+    if (tid < kBatchClasses) {
+        ctl->class_groups[tid] = gbase[tid];
+        ctl->queue[0][tid] = 0;
+        ctl->queue[1][tid] = 0;
+    }
+    if (tid <= kBatchClasses)
+        ctl->class_docs[tid] = (int32_t)dbase[tid];
+}
+
+// Work lists: (document, group) entries of every active counted document.
+__global__ void __launch_bounds__(kThreads) batch_expand_kernel(BatchRound a)
+{
+    if (a.ctl->overflow)
+        return;
+    for (int i = blockIdx.x; i < a.num_docs; i += gridDim.x) {
+        const BatchDocState st = a.states[i];
+        if (st.npairs <= a.round)
+            continue;
+        const int cls = a.pairs[st.pair0 + a.round].cls;
+        if (cls >= kBatchClasses)
+            continue;
+        const BatchDoc bd = a.docs[i];
+        BatchGroup *list = a.groups[cls];
+        for (int g = threadIdx.x; g < bd.ngroups; g += kThreads)
+            list[bd.group0 + g] = BatchGroup{i, g};
+    }
+}
+
+// Persistent count or emit over class c's work queue.  Thread 0 claims the next
+// item while the current one runs; its latency overlaps the group's loads.
+template <int MODE, int LT, int GT, bool EMIT>
+__global__ void __launch_bounds__(kThreads) batch_work_kernel(BatchRound a, int c)
+{
+    constexpr int W = MODE == kModeTable ? LT : 1;
+    __shared__ uint32_t s_item;
+    const int tid = threadIdx.x;
+    if (a.ctl->overflow)
+        return;
+    const int64_t total = a.ctl->class_groups[c];
+    uint32_t *head = &a.ctl->queue[EMIT ? 1 : 0][c];
+    if (tid == 0)
+        s_item = atomicAdd(head, 1u);
+    __syncthreads();
+    int64_t item = s_item;
+    __syncthreads();
+    while (item < total) {
+        uint32_t nxt = 0;
+        if (tid == 0)
+            nxt = atomicAdd(head, 1u);
+        const BatchGroup bg = a.groups[c][item];
+        const ReplaceParams p = doc_view(a.docs[bg.doc], a.pool);
+        if constexpr (EMIT)
+            emit_group<MODE, LT, GT>(p, bg.g, a.pre[c] + item);
+        else
+            count_group<MODE, LT, GT>(p, bg.g, a.gsum[c] + item * W);
+        __syncthreads();   // the group functions' LDS and s_item are reused next
+        if (tid == 0)
+            s_item = nxt;
+        __syncthreads();
+        item = s_item;
+        __syncthreads();
+    }
+}
+
+// Persistent spine: one wave per document of class c.
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kWave) batch_round_spine_kernel(BatchRound a, int c)
+{
+    if (a.ctl->overflow)
+        return;
+    const int nd = a.ctl->class_docs[c];
+    for (int k = blockIdx.x; k < nd; k += gridDim.x)
+        batch_spine_doc<MODE, LT>(a.docs, a.gsum[c], a.pre[c], a.doc_kept, a.cdocs[c][k]);
+}
+
+// Placement: each active document's new length and destination, packed at
+// 256-byte boundaries into the destination arena or, after its last pair, the
+// final arena; updates the document states for the next round.
+__global__ void __launch_bounds__(kPlanThreads) batch_place_kernel(BatchRound a)
+{
+    __shared__ int64_t s_w[kPlanWaves];
+    const int tid = threadIdx.x;
+    BatchCtl *ctl = a.ctl;
+    if (ctl->overflow)
+        return;
+    int64_t fin_base = ctl->fin_used, arena_base = 0, algo = 0;
+    bool over = false;
+    for (int base = 0; base < a.num_docs; base += kPlanThreads) {
+        const int i = base + tid;
+        BatchDocState st{};
+        bool active = false, last = false;
+        int64_t nl = 0;
+        if (i < a.num_docs) {
+            st = a.states[i];
+            if (st.npairs > a.round) {
+                active = true;
+                last = st.npairs == a.round + 1;
+                const BatchPair pr = a.pairs[st.pair0 + a.round];
+                nl = pr.cls == kBatchEmpty ? st.len + (st.len + 1) * (int64_t)pr.r
+                                           : st.len + a.doc_kept[i] * ((int64_t)pr.r - pr.L);
+            }
+        }
+        const int64_t sz = active ? (nl + 255) / 256 * 256 : 0;
+        int64_t tf, ta;
+        const int64_t exf = block_excl_scan_i64(last ? sz : 0, s_w, &tf);
+        const int64_t exa = block_excl_scan_i64(active && !last ? sz : 0, s_w, &ta);
+        if (active) {
+            uint8_t *dst;
+            if (last) {
+                st.fin_off = fin_base + exf;
+                over |= st.fin_off + sz > a.fin_cap;
+                dst = a.fin + st.fin_off;
+            } else {
+                st.off = arena_base + exa;
+                over |= st.off + sz > a.dst_cap;
+                dst = a.dst_arena + st.off;
+            }
+            algo += st.len + nl;
+            st.len = nl;
+            a.states[i] = st;
+            a.docs[i].dst = dst;
+        }
+        fin_base += tf;
+        arena_base += ta;
+    }
+    if (over)
+        ctl->overflow = 1;
+    int64_t algo_total;
+    (void)block_excl_scan_i64(algo, s_w, &algo_total);
+    if (tid == 0) {
+        ctl->fin_used = fin_base;
+        ctl->algo_bytes += algo_total;
+    }
+}
+
+// Empty-pattern documents of the round: the replacement at every gap.
+__global__ void __launch_bounds__(kThreads) batch_empty_kernel(BatchRound a)
+{
+    if (a.ctl->overflow)
+        return;
+    const int nd = a.ctl->class_docs[kBatchEmpty];
+    for (int k = blockIdx.x; k < nd; k += gridDim.x) {
+        const BatchDoc d = a.docs[a.cdocs[kBatchEmpty][k]];
+        const uint8_t *rep = a.pool + d.rep_off;
+        const int64_t r = d.r;
+        for (int64_t i = threadIdx.x; i <= d.n; i += kThreads) {
+            uint8_t *o = d.dst + i * (r + 1);
+            for (int64_t q = 0; q < r; ++q)
+                o[q] = rep[q];
+            if (i < d.n)
+                o[r] = d.src[i];
+        }
+    }
 }
 
 // ---- fused multi-pattern pass ------------------------------------------------
@@ -1890,6 +2166,59 @@ hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int g
         hipLaunchKernelGGL((batch_emit_kernel<M, LT, GT>), dim3((unsigned)groups), dim3(kThreads), 0, s, a);
         return hipGetLastError();
     });
+}
+
+hipError_t launch_batch_round(const BatchRound &a, const BatchRoundShape &shape, hipStream_t s)
+{
+    static const int kClassMode[kBatchClasses] = {kModeBorderFree, kModeTable, kModeTable, kModeTable, kModeTable};
+    static const int kClassLt[kBatchClasses] = {1, 2, 4, 8, 16};
+    hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, a);
+    int counted = 0;
+    for (int c = 0; c < kBatchClasses; ++c)
+        counted += shape.present[c] ? shape.class_docs[c] : 0;
+    if (counted > 0) {
+        const int eg = counted < 1024 ? counted : 1024;
+        hipLaunchKernelGGL(batch_expand_kernel, dim3((unsigned)eg), dim3(kThreads), 0, s, a);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    for (int c = 0; c < kBatchClasses && e == hipSuccess; ++c) {
+        if (!shape.present[c])
+            continue;
+        const unsigned grid = (unsigned)shape.work_grid[c];
+        e = dispatch_mode(kClassMode[c], kClassLt[c], a.gt, [&]<int M, int LT, int GT>() {
+            hipLaunchKernelGGL((batch_work_kernel<M, LT, GT, false>), dim3(grid), dim3(kThreads), 0, s, a, c);
+            return hipGetLastError();
+        });
+        if (e != hipSuccess)
+            return e;
+        const int nd = shape.class_docs[c];
+        const unsigned sg = (unsigned)(nd < 2048 ? nd : 2048);
+        e = dispatch_mode(kClassMode[c], kClassLt[c], kGroupTiles, [&]<int M, int LT, int GT>() {
+            hipLaunchKernelGGL((batch_round_spine_kernel<M, LT>), dim3(sg), dim3(kWave), 0, s, a, c);
+            return hipGetLastError();
+        });
+    }
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(batch_place_kernel, dim3(1), dim3(kPlanThreads), 0, s, a);
+    for (int c = 0; c < kBatchClasses && e == hipSuccess; ++c) {
+        if (!shape.present[c])
+            continue;
+        const unsigned grid = (unsigned)shape.work_grid[c];
+        e = dispatch_mode(kClassMode[c], kClassLt[c], a.gt, [&]<int M, int LT, int GT>() {
+            hipLaunchKernelGGL((batch_work_kernel<M, LT, GT, true>), dim3(grid), dim3(kThreads), 0, s, a, c);
+            return hipGetLastError();
+        });
+    }
+    if (e != hipSuccess)
+        return e;
+    if (shape.present[kBatchEmpty]) {
+        const int nd = shape.class_docs[kBatchEmpty];
+        hipLaunchKernelGGL(batch_empty_kernel, dim3((unsigned)(nd < 1024 ? nd : 1024)), dim3(kThreads), 0, s, a);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, int64_t first_group, int64_t groups,

@@ -45,6 +45,7 @@ REQUIRED_SYMBOLS = (
     "cr_batch_create", "cr_batch_run", "cr_batch_destroy", "cr_stream_replace",
     "cr_server_create", "cr_server_submit", "cr_server_wait", "cr_server_release", "cr_server_destroy",
     "cr_apply_batch_ex", "cr_batch_fusable", "cr_autotune_set", "cr_autotune_reset", "cr_autotune_report",
+    "cr_batch_set_mode", "cr_batch_info",
 )
 
 
@@ -154,6 +155,10 @@ class CudaReplaceLib:
         lib.cr_batch_run.restype = C.c_int
         lib.cr_batch_destroy.argtypes = [c_void_p]
         lib.cr_batch_destroy.restype = None
+        lib.cr_batch_set_mode.argtypes = [c_void_p, C.c_int]
+        lib.cr_batch_set_mode.restype = C.c_int
+        lib.cr_batch_info.argtypes = [c_void_p, C.POINTER(C.c_double), C.c_int]
+        lib.cr_batch_info.restype = C.c_int
         lib.cr_stream_replace.argtypes = [c_void_p, C.c_char_p, c_size_t, c_char_pp, c_int_p, c_char_pp, c_int_p,
                                           C.c_int, c_size_t, C.POINTER(c_uint8_p), c_size_t_p,
                                           C.POINTER(C.c_double)]
@@ -362,12 +367,17 @@ class BatchEngine:
     result equals applying the document's pairs in order with ``bytes.replace``.
     The engine keeps its device arenas and pinned staging between runs."""
 
-    def __init__(self, lib: CudaReplaceLib):
+    def __init__(self, lib: CudaReplaceLib, host_rounds: bool = False):
+        """host_rounds=True keeps every run on the host-driven rounds (one readback
+        of kept counts per round) instead of the device-driven, graph-captured
+        rounds used by default."""
         self.lib = lib
         self.h = c_void_p()
         rc = lib.lib.cr_batch_create(C.byref(self.h))
         if rc != 0:
             raise RuntimeError(f"cr_batch_create failed rc={rc}")
+        if host_rounds and lib.lib.cr_batch_set_mode(self.h, 1) != 0:
+            raise RuntimeError("cr_batch_set_mode failed")
         self.last_stats = None
 
     def run(self, docs: Sequence[bytes], pairs: Sequence[Sequence[Tuple[bytes, bytes]]],
@@ -396,8 +406,12 @@ class BatchEngine:
                                        rep_lens, C.byref(base), offs, lens, stats)
         if rc != 0:
             raise RuntimeError(f"cr_batch_run failed rc={rc}")
+        info = (C.c_double * 6)()
+        self.lib.lib.cr_batch_info(self.h, info, 6)
         self.last_stats = {"device_ms": stats[0], "wall_ms": stats[1], "algo_bytes": stats[2],
-                           "rounds": int(stats[3])}
+                           "rounds": int(stats[3]), "path": "device" if info[0] == 2 else "host",
+                           "group_tiles": int(info[1]), "graph_replayed": bool(info[2]),
+                           "tuned": bool(info[3]), "graph": bool(info[5])}
         addr = C.cast(base, c_void_p).value or 0
         if not copy:
             total = max((offs[i] + lens[i] for i in range(n)), default=0)
