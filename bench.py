@@ -16,8 +16,10 @@ roofline = algorithmic bytes of a border-free pass (n_in + n_out) over the pass'
 cpu_baseline = bytes.replace, single thread, on the same three inputs (one pass
            each, best of two), plus a fork-based per-core chunked variant.
 batched  = the uneven-documents workload (8192 documents, log-normal sizes, 1-16
-           pairs each) through the batch engine, host to host, every result checked
-           against the oracle; beside it the per-call loop of the reference's callers
+           pairs each) through the batch engine (device-driven rounds replayed as a
+           HIP graph), host to host, every result checked against the oracle; the
+           same batch through host-driven rounds; beside it the per-call loop of the
+           reference's callers
            (one cuda_replace_unified per document and pair) timed on every 32nd
            document.  Reported as an extra field of the same line.
 serving  = open-loop Poisson arrivals (1000 requests/s for --serve-seconds) of
@@ -205,7 +207,8 @@ def batch_bench(lib, ro, num_docs: int) -> dict:
         f"{sum(len(p) for p in pairs)} pairs, generated in {time.time() - t0:.1f}s")
     eng = BatchEngine(lib)
     eng.run(docs[:256], pairs[:256])                    # warm the engine's buffers
-    eng.run(docs, pairs)
+    eng.run(docs, pairs)                                # group size measured
+    eng.run(docs, pairs)                                # the repeated shape is captured as a graph
     t = time.perf_counter()
     out = eng.run(docs, pairs, copy=False)      # results as views of the pinned output
     wall = time.perf_counter() - t
@@ -214,8 +217,20 @@ def batch_bench(lib, ro, num_docs: int) -> dict:
     exact = all(out[i] == ro.apply_batch(docs[i], pairs[i]) for i in range(num_docs))
     del out
     eng.close()
-    log(f"[bench] batch: {wall * 1e3:.1f} ms host-to-host, bit_exact={exact} "
+    log(f"[bench] batch: {wall * 1e3:.1f} ms host-to-host, {st['device_ms']:.2f} ms device, path={st['path']} "
+        f"graph_replayed={st['graph_replayed']} group_tiles={st['group_tiles']}, bit_exact={exact} "
         f"(oracle {time.perf_counter() - t:.1f}s)")
+    # the same batch through host-driven rounds (one kept-count readback per round)
+    heng = BatchEngine(lib, host_rounds=True)
+    heng.run(docs, pairs, copy=False)
+    t = time.perf_counter()
+    hout = heng.run(docs, pairs, copy=False)
+    hwall = time.perf_counter() - t
+    hst = dict(heng.last_stats)
+    h_exact = all(hout[i] == ro.apply_batch(docs[i], pairs[i]) for i in range(0, num_docs, 64))
+    del hout
+    heng.close()
+    log(f"[bench] batch host-driven rounds: {hwall * 1e3:.1f} ms host-to-host, {hst['device_ms']:.2f} ms device")
     sample = list(range(0, num_docs, 32))
     t = time.perf_counter()
     for i in sample:
@@ -238,9 +253,14 @@ def batch_bench(lib, ro, num_docs: int) -> dict:
         "device_ms": round(st["device_ms"], 3),
         "device_algo_GBps": round(st["algo_bytes"] / (st["device_ms"] / 1e3) / 1e9, 1),
         "rounds": st["rounds"],
+        "path": st["path"],
+        "graph_replayed": st["graph_replayed"],
+        "group_tiles": st["group_tiles"],
+        "host_driven_rounds": {"host_to_host_ms": round(hwall * 1e3, 2), "device_ms": round(hst["device_ms"], 3),
+                               "bit_exact_sample": h_exact},
         "per_call_sample": f"every 32nd document ({len(sample)} docs), one cuda_replace_unified "
                            f"per document and pair",
-        "bit_exact": exact,
+        "bit_exact": exact and h_exact,
     }
 
 
@@ -259,8 +279,11 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
     check = set(int(i) for i in rng.choice(np.flatnonzero(sizes <= (8 << 20)), size=min(40, nreq), replace=False))
     lag = np.zeros(nreq)
     tickets = []
+    import gc
+    gc.collect()
     with Server(lib, hbm_budget=200 << 30, workers=workers) as srv:
         srv.submit_at(text, 0, 1 << 20, pairs[0]).result()      # warm the engines
+        gc.disable()   # the load generator's own pauses must not delay arrivals
         t0 = time.perf_counter()
         for i in range(nreq):
             target = t0 + float(arrivals[i])
@@ -269,7 +292,9 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
                 time.sleep(target - now)
             lag[i] = max(0.0, time.perf_counter() - target)
             tickets.append(srv.submit_at(text, offs[i], int(sizes[i]), pairs[i]))
+        gc.enable()
         lat = np.zeros(nreq)
+        qms = np.zeros(nreq)
         exact = True
         batch_docs = []
         for i, t in enumerate(tickets):
@@ -278,6 +303,7 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
                 o = offs[i]
                 exact &= bytes(view) == ro.apply_batch(text[o:o + int(sizes[i])], pairs[i])
             lat[i] = lag[i] * 1e3 + t.times["latency_ms"]
+            qms[i] = t.times["queue_ms"]
             batch_docs.append(t.times["batch_docs"])
             t.release()
         wall = time.perf_counter() - t0
@@ -291,6 +317,7 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "mean_batch_docs": round(float(np.mean(batch_docs)), 1),
+        "queue_ms_p50_p99": [round(float(np.percentile(qms, 50)), 3), round(float(np.percentile(qms, 99)), 3)],
         "max_submit_lag_ms": round(float(lag.max() * 1e3), 3),
         "offered_GBps": round(float(sizes.sum()) / float(arrivals[-1]) / 1e9, 3),
         "achieved_GBps": round(float(sizes.sum()) / wall / 1e9, 3),

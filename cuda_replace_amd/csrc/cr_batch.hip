@@ -20,6 +20,7 @@
 #include "../../include/cuda_replace.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <map>
 #include <mutex>
@@ -165,13 +166,14 @@ struct Engine {
     // device-driven path
     int mode = 0;   // 0 automatic, 1 host-driven rounds only, 2 device-driven when the pairs allow
     DevBuf g_pairs, g_pool, g_states, g_states0, g_ctl, g_ctl0, g_docs, g_kept;
-    DevBuf g_groups[kBatchClasses], g_gsum[kBatchClasses], g_pre[kBatchClasses], g_cdocs[kBatchClasses + 1];
-    HostBuf h_pairs, h_states, h_ctl;
+    DevBuf g_groups[kBatchClasses], g_gsum[kBatchClasses], g_pre[kBatchClasses], g_rlist;
+    HostBuf h_pairs, h_states, h_ctl, h_rlist;
     size_t hint_arena = 0, hint_fin = 0;       // sizes learnt from earlier runs
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    std::vector<int64_t> graph_key;
-    double info[6] = {0, 0, 0, 0, 0, 0};       // last run: path, tiles per group, graph reused, tuned, rounds
+    std::vector<int64_t> graph_key;            // shape of the captured run
+    std::vector<int64_t> last_key;             // shape of the previous run
+    double info[7] = {0, 0, 0, 0, 0, 0, 0};    // last run (see cr_batch_info)
 };
 
 struct DeviceGuard {
@@ -632,23 +634,56 @@ int batch_bucket(size_t bytes)
     return 3;
 }
 
-BatchRoundShape round_shape(const std::vector<DocState> &st, const std::vector<int8_t> &pcls, int j, int gt,
-                            int grid_max)
+// Round j's classes, document counts and estimated items per class: the groups of
+// each document at the smaller of its worst-case length (every possible occurrence
+// of every pair so far kept) and 1.5x its input length (at least the input
+// length).  Items past the estimate are still done, by the round's persistent rest
+// kernels.
+BatchRoundShape round_shape(const std::vector<DocState> &st, const std::vector<int64_t> &ub_round,
+                            const std::vector<int8_t> &pcls, int j, int gt, int cus)
 {
     BatchRoundShape sh{};
     int64_t est[kBatchClasses] = {0, 0, 0, 0, 0};
-    for (const DocState &d : st) {
+    for (size_t i = 0; i < st.size(); ++i) {
+        const DocState &d = st[i];
         if (d.npairs <= j)
             continue;
         const int c = pcls[(size_t)(d.pair0 + j)];
         sh.present[c] = 1;
         sh.class_docs[c] += 1;
-        if (c < kBatchClasses)
-            est[c] += ((d.len + kTile - 1) / kTile + gt - 1) / gt + 1;
+        if (c < kBatchClasses) {
+            const int64_t len = std::min(ub_round[i], d.len + d.len / 2);
+            est[c] += ((len + kTile - 1) / kTile + gt - 1) / gt;
+        }
     }
     for (int c = 0; c < kBatchClasses; ++c)
-        sh.work_grid[c] = (int)std::max<int64_t>(1, std::min<int64_t>(est[c], grid_max));
+        sh.work_grid[c] = (int)std::max<int64_t>(1, std::min<int64_t>(est[c], 1 << 22));
+    sh.cus = cus;
     return sh;
+}
+
+// Worst-case length of every document before round j (ub[j][i]), capped at 2^50.
+std::vector<std::vector<int64_t>> length_bounds(const std::vector<DocState> &st, int max_pairs, const int *pat_lens,
+                                                const int *rep_lens)
+{
+    std::vector<std::vector<int64_t>> ub((size_t)max_pairs, std::vector<int64_t>(st.size(), 0));
+    const int64_t kCap = 1ll << 50;
+    for (size_t i = 0; i < st.size(); ++i) {
+        int64_t u = st[i].len;
+        for (int j = 0; j < max_pairs; ++j) {
+            ub[(size_t)j][i] = u;
+            if (j >= st[i].npairs)
+                continue;
+            const int64_t q = st[i].pair0 + j;
+            const int64_t L = pat_lens[q], r = rep_lens[q];
+            if (L == 0)
+                u = u > kCap / (r + 1) ? kCap : u + (u + 1) * r;
+            else if (r > L)
+                u = u / L > kCap / (r - L) ? kCap : u + (u / L) * (r - L);
+            u = std::min(u, kCap);
+        }
+    }
+    return ub;
 }
 
 bool valid_run(int num_docs, const uint8_t *const *docs, const size_t *doc_lens, const int *pair_counts,
@@ -723,16 +758,23 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     CK(dev_reserve(E->arena[0], cap_a, false, 0, s));
     CK(dev_reserve(E->arena[1], cap_a, false, 0, s));
     CK(dev_reserve(E->fin, fin0 + std::max(grow_hint, E->hint_fin), false, 0, s));
-    const int64_t group_cap = (int64_t)(std::max(E->arena[0].cap, E->arena[1].cap) / kTile) + num_docs + 1;
+    int64_t group_cap = (int64_t)(std::max(E->arena[0].cap, E->arena[1].cap) / kTile) + num_docs + 1;
     static const int kWords[kBatchClasses] = {1, 2, 4, 8, 16};
-    for (int c = 0; c < kBatchClasses; ++c) {
-        CK(dev_reserve(E->g_groups[c], (size_t)group_cap * sizeof(BatchGroup), false, 0, s));
-        CK(dev_reserve(E->g_gsum[c], (size_t)group_cap * kWords[c] * sizeof(uint32_t), false, 0, s));
-        CK(dev_reserve(E->g_pre[c], (size_t)group_cap * sizeof(ChunkState), false, 0, s));
-    }
+    auto reserve_lists = [&]() -> hipError_t {
+        for (int c = 0; c < kBatchClasses; ++c) {
+            hipError_t e = dev_reserve(E->g_groups[c], (size_t)group_cap * sizeof(BatchGroup), false, 0, s);
+            if (e == hipSuccess)
+                e = dev_reserve(E->g_gsum[c], (size_t)group_cap * kWords[c] * sizeof(uint32_t), false, 0, s);
+            if (e == hipSuccess)
+                e = dev_reserve(E->g_pre[c], (size_t)group_cap * sizeof(ChunkState), false, 0, s);
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
+    };
+    CK(reserve_lists());
     const size_t D = (size_t)std::max(num_docs, 1);
-    for (int c = 0; c <= kBatchClasses; ++c)
-        CK(dev_reserve(E->g_cdocs[c], D * sizeof(int32_t), false, 0, s));
+    CK(dev_reserve(E->g_rlist, (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(int32_t), false, 0, s));
     CK(dev_reserve(E->g_docs, D * sizeof(BatchDoc), false, 0, s));
     CK(dev_reserve(E->g_kept, D * sizeof(int64_t), false, 0, s));
     CK(dev_reserve(E->g_states, D * sizeof(BatchDocState), false, 0, s));
@@ -747,6 +789,28 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     CK(host_reserve(E->h_pairs, (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(BatchPair) + pool + 64));
     CK(host_reserve(E->h_states, D * sizeof(BatchDocState) + sizeof(BatchCtl) + 64));
     CK(host_reserve(E->h_ctl, sizeof(BatchCtl) + 64));
+    CK(host_reserve(E->h_rlist, (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(int32_t) + 64));
+    // each round's active documents grouped by class (classes and activity follow
+    // from the pairs alone)
+    std::vector<int64_t> rbase((size_t)max_pairs + 1, 0);
+    std::vector<std::array<int32_t, kBatchClasses + 2>> rseg((size_t)max_pairs);
+    {
+        int32_t *hl = reinterpret_cast<int32_t *>(E->h_rlist.p);
+        int64_t fill = 0;
+        for (int j = 0; j < max_pairs; ++j) {
+            rbase[(size_t)j] = fill;
+            for (int c = 0; c <= kBatchClasses; ++c) {
+                rseg[(size_t)j][(size_t)c] = (int32_t)(fill - rbase[(size_t)j]);
+                for (int i = 0; i < num_docs; ++i) {
+                    const DocState &d = st[(size_t)i];
+                    if (d.npairs > j && pcls[(size_t)(d.pair0 + j)] == c)
+                        hl[fill++] = i;
+                }
+            }
+            rseg[(size_t)j][kBatchClasses + 1] = (int32_t)(fill - rbase[(size_t)j]);
+        }
+        rbase[(size_t)max_pairs] = fill;
+    }
     copy_pieces(E->h_in.p, pieces, total_in);
     BatchPair *hp = reinterpret_cast<BatchPair *>(E->h_pairs.p);
     uint8_t *hpool = E->h_pairs.p + (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(BatchPair);
@@ -785,6 +849,9 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     if (num_docs > 0)
         CK(hipMemcpyAsync(E->g_states.p, hs, (size_t)num_docs * sizeof(BatchDocState), hipMemcpyHostToDevice, s));
     CK(hipMemcpyAsync(E->g_ctl.p, hc, sizeof(BatchCtl), hipMemcpyHostToDevice, s));
+    if (total_pairs > 0)
+        CK(hipMemcpyAsync(E->g_rlist.p, E->h_rlist.p, (size_t)total_pairs * sizeof(int32_t), hipMemcpyHostToDevice,
+                          s));
     for (int i = 0; i < num_docs; ++i) {
         const DocState &d = st[(size_t)i];
         if (d.npairs == 0 && d.len > 0)
@@ -794,33 +861,39 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
 
     // ---- rounds -------------------------------------------------------------------
     BatchRound base{};
+    auto refresh = [&]() {   // buffer addresses and sizes (after any growth)
+        for (int c = 0; c < kBatchClasses; ++c) {
+            base.groups[c] = reinterpret_cast<BatchGroup *>(E->g_groups[c].p);
+            base.gsum[c] = reinterpret_cast<uint32_t *>(E->g_gsum[c].p);
+            base.pre[c] = reinterpret_cast<ChunkState *>(E->g_pre[c].p);
+        }
+        base.fin = E->fin.p;
+        base.fin_cap = (int64_t)E->fin.cap;
+        base.group_cap = group_cap;
+    };
     base.ctl = reinterpret_cast<BatchCtl *>(E->g_ctl.p);
     base.states = reinterpret_cast<BatchDocState *>(E->g_states.p);
     base.pairs = reinterpret_cast<const BatchPair *>(E->g_pairs.p);
     base.pool = E->g_pool.p;
     base.docs = reinterpret_cast<BatchDoc *>(E->g_docs.p);
-    for (int c = 0; c < kBatchClasses; ++c) {
-        base.groups[c] = reinterpret_cast<BatchGroup *>(E->g_groups[c].p);
-        base.gsum[c] = reinterpret_cast<uint32_t *>(E->g_gsum[c].p);
-        base.pre[c] = reinterpret_cast<ChunkState *>(E->g_pre[c].p);
-    }
-    for (int c = 0; c <= kBatchClasses; ++c)
-        base.cdocs[c] = reinterpret_cast<int32_t *>(E->g_cdocs[c].p);
     base.doc_kept = reinterpret_cast<int64_t *>(E->g_kept.p);
-    base.fin = E->fin.p;
-    base.fin_cap = (int64_t)E->fin.cap;
-    base.group_cap = group_cap;
     base.num_docs = num_docs;
+    refresh();
     auto round_args = [&](int j, int gt) {
         BatchRound a = base;
         a.round = j;
         a.gt = gt;
+        a.rlist = reinterpret_cast<const int32_t *>(E->g_rlist.p) + rbase[(size_t)j];
+        a.rcount = (int32_t)(rbase[(size_t)j + 1] - rbase[(size_t)j]);
+        for (int c = 0; c < kBatchClasses + 2; ++c)
+            a.cseg[c] = rseg[(size_t)j][(size_t)c];
         a.src_arena = E->arena[j & 1].p;
         a.dst_arena = E->arena[(j + 1) & 1].p;
         a.dst_cap = (int64_t)E->arena[(j + 1) & 1].cap;
         return a;
     };
-    const int grid_max = 8 * E->num_cus;
+    const int grid_max = E->num_cus;
+    const std::vector<std::vector<int64_t>> ub = length_bounds(st, max_pairs, pat_lens, rep_lens);
 
     int gt = env_int("CR_GROUP_TILES", 0);
     if (gt != 1 && gt != 2 && gt != 4)
@@ -849,7 +922,8 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
                 CK(hipMemcpyAsync(E->g_states.p, E->g_states0.p, sbytes, hipMemcpyDeviceToDevice, s));
             CK(hipMemcpyAsync(E->g_ctl.p, E->g_ctl0.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));
             CK(hipEventRecord(E->ev0, s));
-            CK(launch_batch_round(round_args(0, kCands[ci]), round_shape(st, pcls, 0, kCands[ci], grid_max), s));
+            CK(launch_batch_round(round_args(0, kCands[ci]), round_shape(st, ub[0], pcls, 0, kCands[ci], grid_max),
+                                  s));
             CK(hipEventRecord(E->ev1, s));
             CK(hipEventSynchronize(E->ev1));
             CK(hipEventElapsedTime(&ms[(size_t)ci], E->ev0, E->ev1));
@@ -870,7 +944,7 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     std::vector<int64_t> key = {num_docs, max_pairs, gt, group_cap, (int64_t)E->fin.cap, (int64_t)E->arena[0].cap,
                                 (int64_t)E->arena[1].cap};
     for (int j = 0; j < max_pairs; ++j) {
-        shapes[(size_t)j] = round_shape(st, pcls, j, gt, grid_max);
+        shapes[(size_t)j] = round_shape(st, ub[(size_t)j], pcls, j, gt, grid_max);
         const BatchRoundShape &sh = shapes[(size_t)j];
         for (int c = 0; c <= kBatchClasses; ++c)
             key.push_back(sh.present[c] ? sh.class_docs[c] : 0);
@@ -886,8 +960,12 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
         key.push_back((int64_t)(uintptr_t)E->g_gsum[c].p);
         key.push_back((int64_t)(uintptr_t)E->g_pre[c].p);
     }
-    for (int c = 0; c <= kBatchClasses; ++c)
-        key.push_back((int64_t)(uintptr_t)E->g_cdocs[c].p);
+    key.push_back((int64_t)(uintptr_t)E->g_rlist.p);
+    for (int j = 0; j <= max_pairs; ++j)
+        key.push_back(rbase[(size_t)j]);
+    for (int j = 0; j < max_pairs; ++j)
+        for (int c = 0; c < kBatchClasses + 2; ++c)
+            key.push_back(rseg[(size_t)j][(size_t)c]);
 
     auto launch_rounds = [&]() -> hipError_t {
         for (int j = 0; j < max_pairs; ++j) {
@@ -897,9 +975,14 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
         }
         return hipSuccess;
     };
+    // a shape is captured when it repeats (a one-off run pays no capture), then
+    // replayed for as long as it keeps coming
     bool reused = false;
-    const bool use_graph = env_int("CR_BATCH_GRAPH", 1) != 0 && max_pairs > 0;
-    if (use_graph && !(E->graph_exec && key == E->graph_key)) {
+    const bool graph_ok = env_int("CR_BATCH_GRAPH", 1) != 0 && max_pairs > 0;
+    const bool replay = graph_ok && E->graph_exec && key == E->graph_key;
+    const bool use_graph = replay || (graph_ok && key == E->last_key);
+    E->last_key = key;
+    if (use_graph && !replay) {
         if (E->graph_exec)
             (void)hipGraphExecDestroy(E->graph_exec);
         if (E->graph)
@@ -936,16 +1019,37 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
         CK(launch_rounds());
     CK(hipEventRecord(E->ev1, s));
 
-    // ---- outcome ----------------------------------------------------------------------
-    CK(hipMemcpyAsync(hc, E->g_ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
+    // ---- outcome: a round that did not fit is resumed with grown buffers ----------------
+    int resumes = 0;
+    for (;; ++resumes) {
+        CK(hipMemcpyAsync(hc, E->g_ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        if (!hc->overflow)
+            break;
+        if (resumes >= 32)
+            return (int)hipErrorOutOfMemory;
+        const int j = hc->overflow - 1;
+        if (hc->need_groups > group_cap) {
+            group_cap = hc->need_groups + hc->need_groups / 4 + num_docs + 1;
+            CK(reserve_lists());
+        }
+        DevBuf &dst = E->arena[(j + 1) & 1];   // round j's destination: nothing live in it
+        if ((size_t)hc->need_arena > dst.cap)
+            CK(dev_reserve(dst, (size_t)hc->need_arena + (size_t)hc->need_arena / 4 + kGranule, false, 0, s));
+        if ((size_t)hc->need_fin > E->fin.cap)
+            CK(dev_reserve(E->fin, (size_t)hc->need_fin + (size_t)hc->need_fin / 4 + kGranule, true,
+                           (size_t)hc->fin_used, s));
+        refresh();
+        hc->overflow = 0;
+        CK(hipMemcpyAsync(E->g_ctl.p, hc, sizeof(BatchCtl), hipMemcpyHostToDevice, s));
+        for (int jj = j; jj < max_pairs; ++jj)
+            CK(launch_batch_round(round_args(jj, gt), shapes[(size_t)jj], s));
+        CK(hipEventRecord(E->ev1, s));
+    }
+    E->hint_arena = std::max(E->arena[0].cap, E->arena[1].cap);
+    E->hint_fin = E->fin.cap - fin0;
     if (num_docs > 0)
         CK(hipMemcpyAsync(hs, E->g_states.p, (size_t)num_docs * sizeof(BatchDocState), hipMemcpyDeviceToHost, s));
-    CK(hipStreamSynchronize(s));
-    if (hc->overflow) {
-        E->hint_arena = 2 * std::max(E->arena[0].cap, E->arena[1].cap);
-        E->hint_fin = 2 * (E->fin.cap - fin0);
-        return kFallback;
-    }
     const size_t fin_used = (size_t)hc->fin_used;
     CK(host_reserve(E->h_out, fin_used + 64));
     if (fin_used > 0)
@@ -970,6 +1074,7 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     E->info[3] = tuned_now ? 1 : 0;
     E->info[4] = max_pairs;
     E->info[5] = use_graph ? 1 : 0;
+    E->info[6] = resumes;
     return 0;
 }
 
@@ -985,16 +1090,17 @@ void engine_free(Engine *E)
     for (DevBuf *b : dev)
         if (b->p)
             (void)hipFree(b->p);
-    for (int c = 0; c <= kBatchClasses; ++c) {
-        DevBuf *per[] = {c < kBatchClasses ? &E->g_groups[c] : nullptr, c < kBatchClasses ? &E->g_gsum[c] : nullptr,
-                         c < kBatchClasses ? &E->g_pre[c] : nullptr, &E->g_cdocs[c]};
+    for (int c = 0; c < kBatchClasses; ++c) {
+        DevBuf *per[] = {&E->g_groups[c], &E->g_gsum[c], &E->g_pre[c]};
         for (DevBuf *b : per)
-            if (b && b->p)
+            if (b->p)
                 (void)hipFree(b->p);
     }
+    if (E->g_rlist.p)
+        (void)hipFree(E->g_rlist.p);
     HostBuf *host[] = {&E->h_in, &E->h_docs[0], &E->h_docs[1], &E->h_groups[0], &E->h_groups[1],
                        &E->h_cdocs[0], &E->h_cdocs[1], &E->h_pool[0], &E->h_pool[1], &E->h_kept,
-                       &E->h_out, &E->h_misc, &E->h_pairs, &E->h_states, &E->h_ctl};
+                       &E->h_out, &E->h_misc, &E->h_pairs, &E->h_states, &E->h_ctl, &E->h_rlist};
     for (HostBuf *b : host)
         if (b->p)
             (void)hipHostFree(b->p);
@@ -1063,7 +1169,7 @@ CR_API int cr_batch_run(void *b, int num_docs, const uint8_t *const *docs, const
     if (rc == 0) {
         E->info[0] = 1;
         E->info[1] = kGroupTiles;
-        E->info[2] = E->info[3] = E->info[5] = 0;
+        E->info[2] = E->info[3] = E->info[5] = E->info[6] = 0;
         E->info[4] = stats ? stats[3] : 0;
     }
     return rc;
@@ -1083,14 +1189,15 @@ CR_API int cr_batch_set_mode(void *b, int mode)
 
 // The last run: info[0] path (1 host-driven rounds, 2 device-driven), [1] tiles per
 // group, [2] HIP graph replayed without capture, [3] group size measured in this
-// run, [4] rounds, [5] run as a HIP graph.  n: entries wanted (up to 6).
+// run, [4] rounds, [5] run as a HIP graph, [6] rounds resumed after growing the
+// buffers.  n: entries wanted (up to 7).
 CR_API int cr_batch_info(void *b, double *info, int n)
 {
     if (!b || !info || n < 0)
         return -1;
     Engine *E = static_cast<Engine *>(b);
     std::lock_guard<std::mutex> lock(E->mtx);
-    for (int i = 0; i < n && i < 6; ++i)
+    for (int i = 0; i < n && i < 7; ++i)
         info[i] = E->info[i];
     return 0;
 }

@@ -178,12 +178,15 @@ hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int g
 // ---- device-driven batch: every round planned and placed on the GPU ----------------
 // Lengths never come back to the host between rounds, so a whole run is a fixed
 // launch sequence (captured once as a HIP graph and replayed).  Per round: plan
-// (one workgroup: each active document's class, group count and group range),
-// expand (the class work lists), per class a persistent count over a work queue and
-// a persistent spine, place (one workgroup: new lengths and destinations, packed in
+// (one workgroup: each active document's descriptor and group range; the round's
+// documents, grouped by class, come from the host, which knows every document's
+// pairs), expand (the class work lists of (document, group) items), per class a
+// count over its work list (one workgroup per item up to the host's estimate, a
+// persistent grid for any items beyond it) and a persistent spine, place (one workgroup: new lengths and destinations, packed in
 // the ping-pong arena or the final arena), per class a persistent emit, and the
-// empty-pattern documents.  A plan or placement that would not fit the buffers sets
-// the overflow flag and every later kernel of the run returns at once.
+// empty-pattern documents.  A plan or placement that would not fit the buffers
+// commits nothing and records the round and the sizes it needs; every later kernel
+// of the run returns at once, and the host grows the buffers and resumes there.
 constexpr int kBatchClasses = 5;       // border-free, carry tables of 2/4/8/16 states
 constexpr int kBatchEmpty = 5;         // class index of empty-pattern pairs
 
@@ -208,11 +211,12 @@ struct BatchDocState {
 
 struct BatchCtl {
     int64_t class_groups[kBatchClasses];
-    int32_t class_docs[kBatchClasses + 1];
-    uint32_t queue[2][kBatchClasses];   // work-queue heads: count, emit
     int64_t fin_used;                   // final arena bytes handed out so far
     int64_t algo_bytes;                 // input + output bytes of every applied pair
-    int32_t overflow;
+    int64_t need_groups;                // on overflow: work-list entries the round needs
+    int64_t need_arena;                 //   destination arena bytes
+    int64_t need_fin;                   //   final arena bytes (all documents so far)
+    int32_t overflow;                   // 0, or 1 + the round that did not fit
     int32_t pad;
 };
 
@@ -223,7 +227,9 @@ struct BatchRound {
     const uint8_t *pool;
     BatchDoc *docs;                         // indexed by document
     BatchGroup *groups[kBatchClasses];      // work lists, group_cap entries each
-    int32_t *cdocs[kBatchClasses + 1];      // documents per class, num_docs entries each
+    const int32_t *rlist;                   // the round's active documents, grouped by class
+    int32_t rcount;                         // entries of rlist
+    int32_t cseg[kBatchClasses + 2];        // class c's documents are rlist[cseg[c], cseg[c + 1])
     uint32_t *gsum[kBatchClasses];          // group summaries (group_cap x words)
     ChunkState *pre[kBatchClasses];         // group states (group_cap)
     int64_t *doc_kept;
@@ -242,7 +248,8 @@ struct BatchRound {
 struct BatchRoundShape {
     int present[kBatchClasses + 1];         // class has documents this round
     int class_docs[kBatchClasses + 1];      // exact, known from the pairs
-    int work_grid[kBatchClasses];           // persistent count / emit workgroups
+    int work_grid[kBatchClasses];           // estimated work items per class: one workgroup each
+    int cus;                                // compute units of the device
 };
 
 // One whole round of a device-driven batch on stream s (capturable).

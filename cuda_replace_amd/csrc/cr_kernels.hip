@@ -28,6 +28,8 @@
 //   otherwise: candidate bitmap, one-wave serial greedy, then the emit kernel scatters.
 #include "cr_internal.h"
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include <limits.h>
@@ -1185,46 +1187,40 @@ __device__ __forceinline__ int64_t block_excl_scan_i64(int64_t v, int64_t *s_w, 
     return before + inc - v;
 }
 
-// Round plan: every active document's class, descriptor and group range in its
-// class's work list, and the class document lists; resets the work queues.
+// Round plan: every active document's descriptor and its group range in its
+// class's work list (one scan over the round's list; class segments restart).
 __global__ void __launch_bounds__(kPlanThreads) batch_plan_kernel(BatchRound a)
 {
     __shared__ int64_t s_w[kPlanWaves];
+    __shared__ int64_t s_seg[kBatchClasses + 2];   // group prefix at each segment start
     const int tid = threadIdx.x;
-    BatchCtl *ctl = a.ctl;
-    int64_t gbase[kBatchClasses] = {0, 0, 0, 0, 0};
-    int64_t dbase[kBatchClasses + 1] = {0, 0, 0, 0, 0, 0};
-    bool over = false;
-    for (int base = 0; base < a.num_docs; base += kPlanThreads) {
-        const int i = base + tid;
-        int cls = -1;
+    if (a.ctl->overflow)
+        return;
+    int64_t run = 0;
+    for (int base = 0; base < a.rcount; base += kPlanThreads) {
+        const int k = base + tid;
+        int i = 0, cls = kBatchEmpty;
         int64_t ng = 0;
         BatchDocState st{};
         BatchPair pr{};
-        if (i < a.num_docs) {
+        if (k < a.rcount) {
+            i = a.rlist[k];
             st = a.states[i];
-            if (st.npairs > a.round) {
-                pr = a.pairs[st.pair0 + a.round];
-                cls = pr.cls;
-                if (cls < kBatchClasses)
-                    ng = ((st.len + kTile - 1) / kTile + a.gt - 1) / a.gt;
-            }
+            pr = a.pairs[st.pair0 + a.round];
+            cls = pr.cls;
+            if (cls < kBatchClasses)
+                ng = ((st.len + kTile - 1) / kTile + a.gt - 1) / a.gt;
         }
-        int64_t g0 = 0, d0 = 0;
-        for (int c = 0; c <= kBatchClasses; ++c) {
-            int64_t tot;
-            if (c < kBatchClasses) {
-                const int64_t ex = block_excl_scan_i64(cls == c ? ng : 0, s_w, &tot);
-                if (cls == c)
-                    g0 = gbase[c] + ex;
-                gbase[c] += tot;
-            }
-            const int64_t exd = block_excl_scan_i64(cls == c ? 1 : 0, s_w, &tot);
-            if (cls == c)
-                d0 = dbase[c] + exd;
-            dbase[c] += tot;
+        int64_t tot;
+        const int64_t ex = run + block_excl_scan_i64(ng, s_w, &tot);
+        if (k < a.rcount) {
+            for (int c = 0; c < kBatchClasses + 2; ++c)
+                if (a.cseg[c] == k)
+                    s_seg[c] = ex;
         }
-        if (cls >= 0) {
+        __syncthreads();
+        if (k < a.rcount) {
+            const int64_t g0 = ex - s_seg[cls];
             BatchDoc bd{};
             bd.src = a.src_arena + st.off;
             bd.dst = nullptr;
@@ -1238,20 +1234,24 @@ __global__ void __launch_bounds__(kPlanThreads) batch_plan_kernel(BatchRound a)
             bd.pat_off = pr.pat_off;
             bd.rep_off = pr.rep_off;
             a.docs[i] = bd;
-            a.cdocs[cls][d0] = i;
-            if (cls < kBatchClasses && g0 + ng > a.group_cap)
-                over = true;
+        }
+        run += tot;
+        __syncthreads();   // s_seg entries of this chunk were read
+    }
+    if (tid < kBatchClasses + 2 && a.cseg[tid] >= a.rcount)
+        s_seg[tid] = run;
+    __syncthreads();
+    if (tid < kBatchClasses)
+        a.ctl->class_groups[tid] = s_seg[tid + 1] - s_seg[tid];
+    if (tid == 0) {
+        int64_t most = 0;
+        for (int c = 0; c < kBatchClasses; ++c)
+            most = s_seg[c + 1] - s_seg[c] > most ? s_seg[c + 1] - s_seg[c] : most;
+        if (most > a.group_cap) {
+            a.ctl->need_groups = most;
+            a.ctl->overflow = a.round + 1;
         }
     }
-    if (over)
-        ctl->overflow = 1;
-    if (tid < kBatchClasses) {
-        ctl->class_groups[tid] = gbase[tid];
-        ctl->queue[0][tid] = 0;
-        ctl->queue[1][tid] = 0;
-    }
-    if (tid <= kBatchClasses)
-        ctl->class_docs[tid] = (int32_t)dbase[tid];
 }
 
 // Work lists: (document, group) entries of every active counted document.
@@ -1259,13 +1259,10 @@ __global__ void __launch_bounds__(kThreads) batch_expand_kernel(BatchRound a)
 {
     if (a.ctl->overflow)
         return;
-    for (int i = blockIdx.x; i < a.num_docs; i += gridDim.x) {
-        const BatchDocState st = a.states[i];
-        if (st.npairs <= a.round)
-            continue;
-        const int cls = a.pairs[st.pair0 + a.round].cls;
-        if (cls >= kBatchClasses)
-            continue;
+    const int counted = a.cseg[kBatchClasses];
+    for (int k = blockIdx.x; k < counted; k += gridDim.x) {
+        const int i = a.rlist[k];
+        const int cls = a.pairs[a.states[i].pair0 + a.round].cls;
         const BatchDoc bd = a.docs[i];
         BatchGroup *list = a.groups[cls];
         for (int g = threadIdx.x; g < bd.ngroups; g += kThreads)
@@ -1273,39 +1270,49 @@ __global__ void __launch_bounds__(kThreads) batch_expand_kernel(BatchRound a)
     }
 }
 
-// Persistent count or emit over class c's work queue.  Thread 0 claims the next
-// item while the current one runs; its latency overlaps the group's loads.
+// Count or emit over class c's work list: item = workgroup, for the first
+// gridDim.x items (the host's estimate of the round's items, sized from the
+// documents' lengths; one item per workgroup keeps the register use, and so the
+// occupancy, of the single-buffer kernels).
 template <int MODE, int LT, int GT, bool EMIT>
-__global__ void __launch_bounds__(kThreads) batch_work_kernel(BatchRound a, int c)
+__global__ void __launch_bounds__(kThreads) batch_item_kernel(BatchRound a, int c)
 {
     constexpr int W = MODE == kModeTable ? LT : 1;
-    __shared__ uint32_t s_item;
-    const int tid = threadIdx.x;
+    if (a.ctl->overflow)
+        return;
+    const int64_t item = blockIdx.x;
+    if (item >= a.ctl->class_groups[c])
+        return;
+    const BatchGroup bg = a.groups[c][item];
+    const ReplaceParams p = doc_view(a.docs[bg.doc], a.pool);
+    if constexpr (EMIT)
+        emit_group<MODE, LT, GT>(p, bg.g, a.pre[c] + item);
+    else
+        count_group<MODE, LT, GT>(p, bg.g, a.gsum[c] + item * W);
+}
+
+// The items past the estimate, if the documents grew beyond it: a persistent
+// grid, each workgroup a contiguous share of [first, total).
+template <int MODE, int LT, int GT, bool EMIT>
+__global__ void __launch_bounds__(kThreads) batch_rest_kernel(BatchRound a, int c, int64_t first)
+{
+    constexpr int W = MODE == kModeTable ? LT : 1;
     if (a.ctl->overflow)
         return;
     const int64_t total = a.ctl->class_groups[c];
-    uint32_t *head = &a.ctl->queue[EMIT ? 1 : 0][c];
-    if (tid == 0)
-        s_item = atomicAdd(head, 1u);
-    __syncthreads();
-    int64_t item = s_item;
-    __syncthreads();
-    while (item < total) {
-        uint32_t nxt = 0;
-        if (tid == 0)
-            nxt = atomicAdd(head, 1u);
+    if (total <= first)
+        return;
+    const int64_t per = (total - first + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = first + (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < total ? lo + per : total;
+    for (int64_t item = lo; item < hi; ++item) {
         const BatchGroup bg = a.groups[c][item];
         const ReplaceParams p = doc_view(a.docs[bg.doc], a.pool);
         if constexpr (EMIT)
             emit_group<MODE, LT, GT>(p, bg.g, a.pre[c] + item);
         else
             count_group<MODE, LT, GT>(p, bg.g, a.gsum[c] + item * W);
-        __syncthreads();   // the group functions' LDS and s_item are reused next
-        if (tid == 0)
-            s_item = nxt;
-        __syncthreads();
-        item = s_item;
-        __syncthreads();
+        __syncthreads();   // the group functions' LDS is reused by the next item
     }
 }
 
@@ -1315,14 +1322,16 @@ __global__ void __launch_bounds__(kWave) batch_round_spine_kernel(BatchRound a, 
 {
     if (a.ctl->overflow)
         return;
-    const int nd = a.ctl->class_docs[c];
+    const int k0 = a.cseg[c], nd = a.cseg[c + 1] - a.cseg[c];
     for (int k = blockIdx.x; k < nd; k += gridDim.x)
-        batch_spine_doc<MODE, LT>(a.docs, a.gsum[c], a.pre[c], a.doc_kept, a.cdocs[c][k]);
+        batch_spine_doc<MODE, LT>(a.docs, a.gsum[c], a.pre[c], a.doc_kept, a.rlist[k0 + k]);
 }
 
 // Placement: each active document's new length and destination, packed at
 // 256-byte boundaries into the destination arena or, after its last pair, the
-// final arena; updates the document states for the next round.
+// final arena; updates the document states for the next round.  A first sweep
+// sums the sizes; if they do not fit, nothing is written, the overflow flag gets
+// the round and the sizes needed, and the host grows the buffers and resumes here.
 __global__ void __launch_bounds__(kPlanThreads) batch_place_kernel(BatchRound a)
 {
     __shared__ int64_t s_w[kPlanWaves];
@@ -1330,36 +1339,64 @@ __global__ void __launch_bounds__(kPlanThreads) batch_place_kernel(BatchRound a)
     BatchCtl *ctl = a.ctl;
     if (ctl->overflow)
         return;
-    int64_t fin_base = ctl->fin_used, arena_base = 0, algo = 0;
-    bool over = false;
-    for (int base = 0; base < a.num_docs; base += kPlanThreads) {
-        const int i = base + tid;
-        BatchDocState st{};
-        bool active = false, last = false;
-        int64_t nl = 0;
-        if (i < a.num_docs) {
-            st = a.states[i];
-            if (st.npairs > a.round) {
-                active = true;
-                last = st.npairs == a.round + 1;
-                const BatchPair pr = a.pairs[st.pair0 + a.round];
-                nl = pr.cls == kBatchEmpty ? st.len + (st.len + 1) * (int64_t)pr.r
-                                           : st.len + a.doc_kept[i] * ((int64_t)pr.r - pr.L);
-            }
+    auto sizes = [&](int k, int *i, BatchDocState *st, bool *last, int64_t *nl) {
+        *i = a.rlist[k];
+        *st = a.states[*i];
+        *last = st->npairs == a.round + 1;
+        const BatchPair pr = a.pairs[st->pair0 + a.round];
+        *nl = pr.cls == kBatchEmpty ? st->len + (st->len + 1) * (int64_t)pr.r
+                                    : st->len + a.doc_kept[*i] * ((int64_t)pr.r - pr.L);
+    };
+    int64_t fin_total = 0, arena_total = 0;
+    for (int base = 0; base < a.rcount; base += kPlanThreads) {
+        const int k = base + tid;
+        int64_t f = 0, ar = 0;
+        if (k < a.rcount) {
+            int i;
+            BatchDocState st;
+            bool last;
+            int64_t nl;
+            sizes(k, &i, &st, &last, &nl);
+            const int64_t sz = (nl + 255) / 256 * 256;
+            f = last ? sz : 0;
+            ar = last ? 0 : sz;
         }
-        const int64_t sz = active ? (nl + 255) / 256 * 256 : 0;
         int64_t tf, ta;
-        const int64_t exf = block_excl_scan_i64(last ? sz : 0, s_w, &tf);
-        const int64_t exa = block_excl_scan_i64(active && !last ? sz : 0, s_w, &ta);
-        if (active) {
+        (void)block_excl_scan_i64(f, s_w, &tf);
+        (void)block_excl_scan_i64(ar, s_w, &ta);
+        fin_total += tf;
+        arena_total += ta;
+    }
+    const int64_t fin0 = ctl->fin_used;
+    if (fin0 + fin_total > a.fin_cap || arena_total > a.dst_cap) {
+        if (tid == 0) {
+            ctl->need_fin = fin0 + fin_total;
+            ctl->need_arena = arena_total;
+            ctl->overflow = a.round + 1;
+        }
+        return;
+    }
+    int64_t fin_base = fin0, arena_base = 0, algo = 0;
+    for (int base = 0; base < a.rcount; base += kPlanThreads) {
+        const int k = base + tid;
+        int i = 0;
+        BatchDocState st{};
+        bool last = false;
+        int64_t nl = 0, sz = 0;
+        if (k < a.rcount) {
+            sizes(k, &i, &st, &last, &nl);
+            sz = (nl + 255) / 256 * 256;
+        }
+        int64_t tf, ta;
+        const int64_t exf = block_excl_scan_i64(k < a.rcount && last ? sz : 0, s_w, &tf);
+        const int64_t exa = block_excl_scan_i64(k < a.rcount && !last ? sz : 0, s_w, &ta);
+        if (k < a.rcount) {
             uint8_t *dst;
             if (last) {
                 st.fin_off = fin_base + exf;
-                over |= st.fin_off + sz > a.fin_cap;
                 dst = a.fin + st.fin_off;
             } else {
                 st.off = arena_base + exa;
-                over |= st.off + sz > a.dst_cap;
                 dst = a.dst_arena + st.off;
             }
             algo += st.len + nl;
@@ -1370,8 +1407,6 @@ __global__ void __launch_bounds__(kPlanThreads) batch_place_kernel(BatchRound a)
         fin_base += tf;
         arena_base += ta;
     }
-    if (over)
-        ctl->overflow = 1;
     int64_t algo_total;
     (void)block_excl_scan_i64(algo, s_w, &algo_total);
     if (tid == 0) {
@@ -1385,9 +1420,9 @@ __global__ void __launch_bounds__(kThreads) batch_empty_kernel(BatchRound a)
 {
     if (a.ctl->overflow)
         return;
-    const int nd = a.ctl->class_docs[kBatchEmpty];
+    const int k0 = a.cseg[kBatchEmpty], nd = a.cseg[kBatchEmpty + 1] - a.cseg[kBatchEmpty];
     for (int k = blockIdx.x; k < nd; k += gridDim.x) {
-        const BatchDoc d = a.docs[a.cdocs[kBatchEmpty][k]];
+        const BatchDoc d = a.docs[a.rlist[k0 + k]];
         const uint8_t *rep = a.pool + d.rep_off;
         const int64_t r = d.r;
         for (int64_t i = threadIdx.x; i <= d.n; i += kThreads) {
@@ -2168,6 +2203,16 @@ hipError_t launch_batch_emit(const BatchArgs &a, int mode, int table_size, int g
     });
 }
 
+template <int M, int LT, int GT, bool EMIT>
+hipError_t launch_work(const BatchRound &a, int c, const BatchRoundShape &shape, hipStream_t s)
+{
+    const int64_t est = shape.work_grid[c] > 0 ? shape.work_grid[c] : 1;
+    hipLaunchKernelGGL((batch_item_kernel<M, LT, GT, EMIT>), dim3((unsigned)est), dim3(kThreads), 0, s, a, c);
+    const unsigned rest = (unsigned)(shape.cus > 0 ? shape.cus : 1);
+    hipLaunchKernelGGL((batch_rest_kernel<M, LT, GT, EMIT>), dim3(rest), dim3(kThreads), 0, s, a, c, est);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_round(const BatchRound &a, const BatchRoundShape &shape, hipStream_t s)
 {
     static const int kClassMode[kBatchClasses] = {kModeBorderFree, kModeTable, kModeTable, kModeTable, kModeTable};
@@ -2186,10 +2231,8 @@ hipError_t launch_batch_round(const BatchRound &a, const BatchRoundShape &shape,
     for (int c = 0; c < kBatchClasses && e == hipSuccess; ++c) {
         if (!shape.present[c])
             continue;
-        const unsigned grid = (unsigned)shape.work_grid[c];
         e = dispatch_mode(kClassMode[c], kClassLt[c], a.gt, [&]<int M, int LT, int GT>() {
-            hipLaunchKernelGGL((batch_work_kernel<M, LT, GT, false>), dim3(grid), dim3(kThreads), 0, s, a, c);
-            return hipGetLastError();
+            return launch_work<M, LT, GT, false>(a, c, shape, s);
         });
         if (e != hipSuccess)
             return e;
@@ -2206,10 +2249,8 @@ hipError_t launch_batch_round(const BatchRound &a, const BatchRoundShape &shape,
     for (int c = 0; c < kBatchClasses && e == hipSuccess; ++c) {
         if (!shape.present[c])
             continue;
-        const unsigned grid = (unsigned)shape.work_grid[c];
         e = dispatch_mode(kClassMode[c], kClassLt[c], a.gt, [&]<int M, int LT, int GT>() {
-            hipLaunchKernelGGL((batch_work_kernel<M, LT, GT, true>), dim3(grid), dim3(kThreads), 0, s, a, c);
-            return hipGetLastError();
+            return launch_work<M, LT, GT, true>(a, c, shape, s);
         });
     }
     if (e != hipSuccess)

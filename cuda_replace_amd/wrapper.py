@@ -406,12 +406,12 @@ class BatchEngine:
                                        rep_lens, C.byref(base), offs, lens, stats)
         if rc != 0:
             raise RuntimeError(f"cr_batch_run failed rc={rc}")
-        info = (C.c_double * 6)()
-        self.lib.lib.cr_batch_info(self.h, info, 6)
+        info = (C.c_double * 7)()
+        self.lib.lib.cr_batch_info(self.h, info, 7)
         self.last_stats = {"device_ms": stats[0], "wall_ms": stats[1], "algo_bytes": stats[2],
                            "rounds": int(stats[3]), "path": "device" if info[0] == 2 else "host",
                            "group_tiles": int(info[1]), "graph_replayed": bool(info[2]),
-                           "tuned": bool(info[3]), "graph": bool(info[5])}
+                           "tuned": bool(info[3]), "graph": bool(info[5]), "resumes": int(info[6])}
         addr = C.cast(base, c_void_p).value or 0
         if not copy:
             total = max((offs[i] + lens[i] for i in range(n)), default=0)

@@ -205,20 +205,22 @@ CR_API int cr_batch_run(void *b, int num_docs, const uint8_t *const *docs, const
 /* How the engine runs rounds.  mode 0 (default; CR_BATCH_DEVICE=0 in the
  * environment makes 1 the default): device-driven rounds whenever no pair needs
  * the bitmap path -- every round is planned, counted, placed and emitted on the
- * GPU by persistent work-queue kernels, the whole run is captured once as a HIP
- * graph and replayed while its shape and the engine's buffers stay the same
- * (CR_BATCH_GRAPH=0 launches the kernels directly), and the tiles per group are
- * measured on the first run of an input size bucket (CR_GROUP_TILES fixes them).
- * A run that does not fit the engine's buffers is redone with host-driven rounds
- * and the buffers grow for the next run.  mode 1: host-driven rounds only (one
- * readback of kept counts per round; also the bitmap-path fallback).  Returns 0 or
- * -1.  No counterpart in the reference. */
+ * GPU by persistent kernels over device-built work lists; a run shape that
+ * repeats is captured as a HIP graph and replayed while the shape and the
+ * engine's buffers stay the same (CR_BATCH_GRAPH=0 launches the kernels
+ * directly); the tiles per group are measured on the first run of an input size
+ * bucket (CR_GROUP_TILES fixes them).  A round that does not fit the engine's
+ * buffers commits nothing; the host grows the buffers and resumes at that round.
+ * mode 1: host-driven rounds only (one readback of kept counts per round; also
+ * used for pairs that need the bitmap path).  Returns 0 or -1.  No counterpart in
+ * the reference. */
 CR_API int cr_batch_set_mode(void *b, int mode);
 
 /* The last run of the engine: info[0] 1 host-driven / 2 device-driven, [1] tiles
  * per group, [2] the HIP graph was replayed without a new capture, [3] the group
- * size was measured in this run, [4] rounds, [5] ran as a HIP graph.  n entries
- * (at most 6) are written.  Returns 0 or -1. */
+ * size was measured in this run, [4] rounds, [5] ran as a HIP graph, [6] times a
+ * round was resumed after the buffers grew.  n entries (at most 7) are written.
+ * Returns 0 or -1. */
 CR_API int cr_batch_info(void *b, double *info, int n);
 
 /* Destroys a batch engine; NULL is ignored. */

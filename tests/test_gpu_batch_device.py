@@ -1,12 +1,13 @@
 """Device-driven batch rounds (cr_batch_run's default path) on the GPU.
 
-Every round is planned, counted, placed and emitted by persistent work-queue
-kernels and the whole run is a captured HIP graph.  Each result must equal
+Every round is planned, counted, placed and emitted by persistent kernels over
+device-built work lists, and a run shape that repeats is captured as a HIP graph
+and replayed.  Each result must equal
 applying the document's pairs in order with bytes.replace (oracle apply_batch):
 for every counted pattern class and the empty pattern, with documents without
 pairs and empty documents mixed in; on graph replays separated by other GPU work
-(each replay checked); after a run that overflows the engine's buffers (redone by
-the host-driven rounds); and for the group size chosen by the autotuner.
+(each replay checked); after a round that overflows the engine's buffers (resumed
+once they grew); and for the group size chosen by the autotuner.
 """
 import random
 
@@ -53,7 +54,9 @@ def test_device_rounds_every_class(lib):
     with BatchEngine(lib) as eng:
         _check(eng, docs, pairs)
         st = eng.last_stats
-        assert st["graph"] and st["rounds"] == max(len(p) for p in pairs)
+        assert not st["graph"] and st["rounds"] == max(len(p) for p in pairs)   # a one-off shape
+        _check(eng, docs, pairs)
+        assert eng.last_stats["graph"] and not eng.last_stats["graph_replayed"]  # repeated: captured
 
 
 def test_graph_replays_with_other_work_between(lib):
@@ -62,7 +65,8 @@ def test_graph_replays_with_other_work_between(lib):
     pairs = [[rng.choice(COUNTED) for _ in range(rng.randint(1, 5))] for _ in docs]
     with BatchEngine(lib) as eng, BatchEngine(lib, host_rounds=True) as host:
         first = _check(eng, docs, pairs)
-        assert not eng.last_stats["graph_replayed"]
+        assert _check(eng, docs, pairs) == first
+        assert eng.last_stats["graph"] and not eng.last_stats["graph_replayed"]
         for rep_i in range(3):
             # other GPU work between the replays: a host-driven batch and a session pass
             _check(host, docs[:7], pairs[:7], path="host")
@@ -85,15 +89,16 @@ def test_new_shape_recaptures(lib):
                 assert eng.last_stats["path"] == "device"
 
 
-def test_overflow_is_redone_by_host_rounds(lib):
-    # 1-byte pattern -> 40 bytes: outputs far beyond the first run's buffer sizing
-    docs = [b"a" * 200_000, b"xa" * 3000, b""]
-    pairs = [[(b"a", b"Q" * 40)], [(b"a", b"R" * 40), (b"R", b"ss")], [(b"a", b"b")]]
+def test_overflow_resumes_with_grown_buffers(lib):
+    # 1-byte pattern -> 40 bytes: outputs far beyond the first run's buffer sizing;
+    # the round that does not fit commits nothing and is resumed after the growth
+    docs = [b"a" * 200_000, b"xa" * 3000, b"", b"q" * 9000]
+    pairs = [[(b"a", b"Q" * 40)], [(b"a", b"R" * 40), (b"R", b"ss"), (b"s", b"tt")], [(b"a", b"b")], []]
     with BatchEngine(lib) as eng:
-        _check(eng, docs, pairs, path="host")
-        for _ in range(4):   # buffers double after each overflow until the device rounds fit
-            _check(eng, docs, pairs, path=None)
-        assert eng.last_stats["path"] == "device"
+        _check(eng, docs, pairs)
+        assert eng.last_stats["resumes"] >= 1
+        _check(eng, docs, pairs)
+        assert eng.last_stats["resumes"] == 0   # the engine kept the grown sizes
 
 
 def test_bitmap_pairs_use_host_rounds(lib):
