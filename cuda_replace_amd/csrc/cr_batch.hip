@@ -618,8 +618,9 @@ int env_int(const char *name, int dflt)
 }
 
 // Tiles per group for device-driven runs, per device and input size bucket: the
-// first run of a bucket executes round 0 once per candidate (1, 2, 4) from the same
-// starting state, timed with HIP events, and keeps the fastest.
+// first run of a bucket executes round 0 for each candidate (1, 2, 4) from the same
+// starting state, in three interleaved sweeps timed with HIP events, and keeps the
+// fastest (another size than 2 only when more than 3% faster).
 std::mutex g_tune_mtx;
 std::map<std::pair<int, int>, std::pair<int, std::vector<float>>> g_tune;
 
@@ -915,23 +916,26 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
             CK(hipMemcpyAsync(E->g_states0.p, E->g_states.p, sbytes, hipMemcpyDeviceToDevice, s));
         CK(hipMemcpyAsync(E->g_ctl0.p, E->g_ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));
         std::vector<float> ms(3, 0.f);
-        float best = 0.f;
         static const int kCands[3] = {1, 2, 4};
-        for (int ci = 0; ci < 3; ++ci) {
-            if (sbytes > 0)
-                CK(hipMemcpyAsync(E->g_states.p, E->g_states0.p, sbytes, hipMemcpyDeviceToDevice, s));
-            CK(hipMemcpyAsync(E->g_ctl.p, E->g_ctl0.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));
-            CK(hipEventRecord(E->ev0, s));
-            CK(launch_batch_round(round_args(0, kCands[ci]), round_shape(st, ub[0], pcls, 0, kCands[ci], grid_max),
-                                  s));
-            CK(hipEventRecord(E->ev1, s));
-            CK(hipEventSynchronize(E->ev1));
-            CK(hipEventElapsedTime(&ms[(size_t)ci], E->ev0, E->ev1));
-            if (gt == 0 || ms[(size_t)ci] < best) {
-                gt = kCands[ci];
-                best = ms[(size_t)ci];
+        for (int sweep = 0; sweep < 3; ++sweep) {   // interleaved, best time per candidate
+            for (int ci = 0; ci < 3; ++ci) {
+                if (sbytes > 0)
+                    CK(hipMemcpyAsync(E->g_states.p, E->g_states0.p, sbytes, hipMemcpyDeviceToDevice, s));
+                CK(hipMemcpyAsync(E->g_ctl.p, E->g_ctl0.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));
+                CK(hipEventRecord(E->ev0, s));
+                CK(launch_batch_round(round_args(0, kCands[ci]),
+                                      round_shape(st, ub[0], pcls, 0, kCands[ci], grid_max), s));
+                CK(hipEventRecord(E->ev1, s));
+                CK(hipEventSynchronize(E->ev1));
+                float t = 0.f;
+                CK(hipEventElapsedTime(&t, E->ev0, E->ev1));
+                ms[(size_t)ci] = sweep == 0 || t < ms[(size_t)ci] ? t : ms[(size_t)ci];
             }
         }
+        gt = kGroupTiles;   // another size only when more than 3% faster
+        for (int ci = 0; ci < 3; ++ci)
+            if (ms[(size_t)ci] < 0.97f * ms[1] && ms[(size_t)ci] < ms[(size_t)(gt == 1 ? 0 : gt == 2 ? 1 : 2)])
+                gt = kCands[ci];
         if (sbytes > 0)
             CK(hipMemcpyAsync(E->g_states.p, E->g_states0.p, sbytes, hipMemcpyDeviceToDevice, s));
         CK(hipMemcpyAsync(E->g_ctl.p, E->g_ctl0.p, sizeof(BatchCtl), hipMemcpyDeviceToDevice, s));

@@ -39,9 +39,10 @@ constexpr size_t kAllocGranule = 2u << 20;   // device buffers grow in 2 MiB ste
 // and the per-workgroup fixed costs against LDS per workgroup (occupancy) and tail
 // balance.  The best one is measured, not assumed: the first pass of a pattern class
 // (border-free, carry tables of 2/4/8/16 states) in an input size bucket on a device
-// runs count + spine + emit once per candidate after its normal pass, times each
-// with HIP events (best of two), and records the fastest for every later pass in the
-// bucket.  Inputs under 4 MiB are not tuned.  CR_GROUP_TILES=1|2|4 fixes the choice;
+// runs count + spine + emit per candidate after its normal pass (three interleaved
+// sweeps, HIP events, best time per candidate) and records the fastest for every
+// later pass in the bucket; another size replaces the default 2 only when it is
+// more than 3% faster.  Inputs under 4 MiB are not tuned.  CR_GROUP_TILES=1|2|4 fixes the choice;
 // CR_AUTOTUNE=0 keeps the default.
 constexpr int kTuneCands[3] = {1, 2, 4};
 
@@ -634,14 +635,15 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
         if (tune && num_chunks == 1) {
             // every candidate recomputes the same summaries, states, records and
-            // output, so the pass's results above stay valid
+            // output, so the pass's results above stay valid.  Three interleaved
+            // sweeps (so clock and cache drift hit every candidate alike), best
+            // time per candidate; another size replaces the default only when it
+            // is more than 3% faster.
             TuneEntry te;
-            float best = 0.f;
-            for (int ci = 0; ci < 3; ++ci) {
-                const int cand = kTuneCands[ci];
-                const int64_t ng = groups_of(num_tiles, cand);
-                float tbest = 0.f;
-                for (int rep_i = 0; rep_i < 2; ++rep_i) {
+            for (int sweep = 0; sweep < 3; ++sweep) {
+                for (int ci = 0; ci < 3; ++ci) {
+                    const int cand = kTuneCands[ci];
+                    const int64_t ng = groups_of(num_tiles, cand);
                     CK(hipEventRecord(s->ev0, s->stream));
                     CK(launch_count(p, mode, table_size, cand, 0, ng, s->stream));
                     CK(launch_spine(p, mode, table_size, 0, ng, 0, 1, s->stream));
@@ -650,12 +652,15 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                     CK(hipEventSynchronize(s->ev1));
                     float ms = 0.f;
                     CK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-                    tbest = rep_i == 0 || ms < tbest ? ms : tbest;
+                    te.ms[ci] = sweep == 0 || ms < te.ms[ci] ? ms : te.ms[ci];
                 }
-                te.ms[ci] = tbest;
-                if (te.gt == 0 || tbest < best) {
-                    te.gt = cand;
-                    best = tbest;
+            }
+            te.gt = kGroupTiles;
+            float best = te.ms[1];
+            for (int ci = 0; ci < 3; ++ci) {
+                if (te.ms[ci] < best && te.ms[ci] < 0.97f * te.ms[1]) {
+                    te.gt = kTuneCands[ci];
+                    best = te.ms[ci];
                 }
             }
             record_tuning(s->device, mode, table_size, n, te);

@@ -2,7 +2,8 @@
 workgroup) must give the oracle's output, around the tile and group boundaries of
 each size, for border-free patterns and carry tables of 2/4/8/16 states; and the
 first pass in a tuned size bucket must record a choice with a measured time for
-every candidate, and still return the exact output."""
+every candidate (the default size unless another is more than 3% faster), and
+still return the exact output."""
 import random
 
 import pytest
@@ -69,4 +70,8 @@ def test_first_tuned_pass_measures_every_candidate(lib, pinned):
         assert e["size"] == "4-32MiB" and e["group_tiles"] in (1, 2, 4)
         ms = e["ms"]
         assert all(ms[k] > 0 for k in ("1", "2", "4"))
-        assert ms[str(e["group_tiles"])] == min(ms.values())
+        best = min(ms.values())
+        if e["group_tiles"] == 2:   # the default stays unless another size is >3% faster
+            assert best >= 0.97 * ms["2"] or ms["2"] == best
+        else:
+            assert ms[str(e["group_tiles"])] == best and best < 0.97 * ms["2"]
