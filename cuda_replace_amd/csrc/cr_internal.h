@@ -125,7 +125,8 @@ struct ReplaceParams {
     uint32_t pat4;         // first min(L,4) pattern bytes, little-endian
     uint32_t mask4;        // byte mask of those bytes
     uint32_t *gsum;        // per-group kept count, or LT table entries per group
-    ChunkState *chunks;    // num_chunks + 1 chunk states
+    ChunkState *chunks;    // num_chunks + 1 chunk states (or spine block states)
+    uint64_t *block_tabs;  // spine blocks: each block's summary (kMaxTablePat words)
     ChunkState *chunks_pre; // per group: kept starts before it, carry into it
     TileInfo *info;        // per-tile records (num_tiles)
     uint32_t *bitmap;      // candidate / kept bitmap (bitmap mode)
@@ -264,6 +265,11 @@ hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, 
 // groups count groups of that size.
 hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int gt, int64_t first_group,
                         int64_t groups, hipStream_t s);
+// The spine of a whole pass (one chunk) as three launches: per block of groups its
+// summary, one wave scanning the block summaries (block states in chunks[], the
+// pass total in stats), then every block's groups in parallel.
+hipError_t launch_spine_blocks(const ReplaceParams &p, int mode, int table_size, int64_t groups, hipStream_t s);
+int64_t spine_blocks_needed(int64_t groups);   // chunks[] entries and block_tabs blocks it uses
 hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_t first_group,
                         int64_t groups, int64_t chunk, int last_chunk, hipStream_t s);
 hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int gt, int64_t first_group,

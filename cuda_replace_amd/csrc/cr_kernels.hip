@@ -518,6 +518,16 @@ struct SpineShape {
 };
 
 template <int MODE, int LT>
+constexpr int64_t spine_pass_groups()
+{
+    constexpr int W = MODE == kModeTable ? LT : 1;
+    return (int64_t)kWave * SpineShape<W>::V * kSpineWaves;
+}
+
+// BLOCKS: one workgroup per block of spine_pass_groups() groups (block = chunk =
+// blockIdx.x, its entering state from chunks[]), the pass total published by the
+// block scan instead.
+template <int MODE, int LT, bool BLOCKS = false>
 __global__ void __launch_bounds__(kSpineThreads)
 spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk, int last_chunk)
 {
@@ -528,6 +538,12 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
     __shared__ uint64_t s_wt[kSpineWaves][LT];
     __shared__ int64_t s_wsum[kSpineWaves];
 
+    if constexpr (BLOCKS) {
+        chunk = blockIdx.x;
+        const int64_t total = groups;
+        first_group = chunk * kPass;
+        groups = total - first_group < kPass ? total - first_group : kPass;
+    }
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
@@ -703,7 +719,7 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
         }
         __syncthreads();   // the wave slots are reused by the next pass
     }
-    if (tid == 0) {
+    if (tid == 0 && !BLOCKS) {
         ChunkState cs;
         cs.k_base = k;
         cs.carry = carry;
@@ -712,6 +728,100 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
         if (last_chunk)   // 48-bit two's complement payload (a fused pass's total may be negative)
             p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | ((uint64_t)k & ((1ull << kEpochShift) - 1));
     }
+}
+
+// Spine blocks, first launch: block b's summary over its groups -- the kept count
+// (signed length change for a fused pass) or the composed carry table (entry c:
+// carry out << kCarryShift | kept, for carry in c) -- into block_tabs[b * kMaxTablePat].
+template <int MODE, int LT>
+__global__ void __launch_bounds__(kSpineThreads) spine_reduce_kernel(ReplaceParams p, int64_t groups)
+{
+    constexpr int64_t kPass = spine_pass_groups<MODE, LT>();
+    __shared__ uint64_t s_wt[kSpineWaves][LT];
+    __shared__ int64_t s_wsum[kSpineWaves];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int64_t g0 = (int64_t)blockIdx.x * kPass;
+    const int64_t gn = groups - g0 < kPass ? groups - g0 : kPass;
+    if constexpr (MODE != kModeTable) {
+        int64_t sum = 0;
+        for (int64_t i = tid; i < gn; i += kSpineThreads)
+            sum += MODE == kModeFused ? (int64_t)(int32_t)p.gsum[g0 + i] : (int64_t)p.gsum[g0 + i];
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1)
+            sum += __shfl_xor(sum, d, kWave);
+        if (lane == 0)
+            s_wsum[wave] = sum;
+        __syncthreads();
+        if (tid == 0) {
+            int64_t t = 0;
+            for (int w = 0; w < kSpineWaves; ++w)
+                t += s_wsum[w];
+            p.block_tabs[blockIdx.x * kMaxTablePat] = (uint64_t)t;
+        }
+    } else {
+        // each wave composes a contiguous run of the block in order, then the waves'
+        // tables are composed in wave order
+        constexpr int64_t kPerWave = kPass / kSpineWaves;
+        Table64<LT> run = tidentity64<LT>();
+        const int64_t w0 = (int64_t)wave * kPerWave;
+        for (int64_t b = 0; b < kPerWave; b += kWave) {
+            const int64_t gi = w0 + b + lane;
+            Table<LT> t = tidentity<LT>();
+            if (gi < gn) {
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    t.e[c] = p.gsum[(g0 + gi) * LT + c];
+            }
+            run = tcompose64<LT>(run, wave_broadcast0<LT>(wave_compose<LT>(widen<LT>(t), lane)));
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < LT; ++c)
+                s_wt[wave][c] = run.e[c];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            Table64<LT> tot = tidentity64<LT>();
+            for (int w = 0; w < kSpineWaves; ++w) {
+                Table64<LT> wt;
+#pragma unroll
+                for (int c = 0; c < LT; ++c)
+                    wt.e[c] = s_wt[w][c];
+                tot = tcompose64<LT>(tot, wt);
+            }
+#pragma unroll
+            for (int c = 0; c < LT; ++c)
+                p.block_tabs[blockIdx.x * kMaxTablePat + c] = tot.e[c];
+        }
+    }
+}
+
+// Spine blocks, second launch (one thread): the state entering every block into
+// chunks[b], the state after the last into chunks[nblocks], and the pass total.
+template <int MODE, int LT>
+__global__ void spine_top_kernel(ReplaceParams p, int64_t nblocks)
+{
+    if (threadIdx.x != 0)
+        return;
+    int64_t k = 0;
+    uint32_t carry = 0;
+    for (int64_t b = 0; b <= nblocks; ++b) {
+        ChunkState cs;
+        cs.k_base = k;
+        cs.carry = (int)carry;
+        cs.pad = 0;
+        p.chunks[b] = cs;
+        if (b == nblocks)
+            break;
+        if constexpr (MODE != kModeTable) {
+            k += (int64_t)p.block_tabs[b * kMaxTablePat];
+        } else {
+            const uint64_t e = p.block_tabs[b * kMaxTablePat + carry];
+            k += (int64_t)(e & kCountMask);
+            carry = (uint32_t)(e >> kCarryShift);
+        }
+    }
+    p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | ((uint64_t)k & ((1ull << kEpochShift) - 1));
 }
 
 // ---- emit: resolve kept starts and write each tile's output ----------------
@@ -2072,6 +2182,50 @@ hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int gt
     });
 }
 
+template <int MODE, int LT>
+hipError_t spine_blocks_t(const ReplaceParams &p, int64_t groups, hipStream_t s)
+{
+    constexpr int64_t kPass = spine_pass_groups<MODE, LT>();
+    const int64_t nb = (groups + kPass - 1) / kPass;
+    if (nb <= 1)   // one block: the single-workgroup spine is the whole job
+        return spine_t<MODE, LT>(p, 0, groups, 0, 1, s);
+    hipLaunchKernelGGL((spine_reduce_kernel<MODE, LT>), dim3((unsigned)nb), dim3(kSpineThreads), 0, s, p, groups);
+    hipLaunchKernelGGL((spine_top_kernel<MODE, LT>), dim3(1), dim3(kWave), 0, s, p, nb);
+    hipLaunchKernelGGL((spine_kernel<MODE, LT, true>), dim3((unsigned)nb), dim3(kSpineThreads), 0, s, p, (int64_t)0,
+                       groups, (int64_t)0, 0);
+    return hipGetLastError();
+}
+
+int64_t spine_blocks_needed(int64_t groups)
+{
+    // the smallest block (carry tables of 16 states) has 64 x 1 x 16 groups
+    return groups / ((int64_t)kWave * kSpineWaves) + 2;
+}
+
+hipError_t launch_spine_blocks(const ReplaceParams &p, int mode, int table_size, int64_t groups, hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    if (mode == kModeBorderFree)
+        return spine_blocks_t<kModeBorderFree, 1>(p, groups, s);
+    if (mode == kModeFused)
+        return spine_blocks_t<kModeFused, 1>(p, groups, s);
+    if (mode != kModeTable)
+        return hipErrorInvalidValue;
+    switch (table_size) {
+    case 2:
+        return spine_blocks_t<kModeTable, 2>(p, groups, s);
+    case 4:
+        return spine_blocks_t<kModeTable, 4>(p, groups, s);
+    case 8:
+        return spine_blocks_t<kModeTable, 8>(p, groups, s);
+    case 16:
+        return spine_blocks_t<kModeTable, 16>(p, groups, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_t first_group, int64_t groups,
                         int64_t chunk, int last_chunk, hipStream_t s)
 {
@@ -2270,8 +2424,7 @@ hipError_t launch_fused(const ReplaceParams &p, const FusedTable *t, int which, 
     if (which == 0)
         hipLaunchKernelGGL(fused_count_kernel, dim3((unsigned)groups), dim3(kThreads), 0, s, p, t, first_group);
     else if (which == 1)
-        hipLaunchKernelGGL((spine_kernel<kModeFused, 1>), dim3(1), dim3(kSpineThreads), 0, s, p, first_group, groups,
-                           (int64_t)0, 1);
+        return launch_spine_blocks(p, kModeFused, 1, groups, s);
     else
         hipLaunchKernelGGL(fused_emit_kernel, dim3((unsigned)groups), dim3(kThreads), 0, s, p, t, first_group);
     return hipGetLastError();

@@ -148,8 +148,10 @@ struct Session {
 
     uint32_t *d_gsum = nullptr;      // group summaries (count or carry table per group)
     size_t gsum_words = 0;
-    ChunkState *d_chunks = nullptr;  // state after each chunk
+    ChunkState *d_chunks = nullptr;  // state after each chunk (or entering each spine block)
     size_t chunk_cap = 0;
+    uint64_t *d_blocks = nullptr;    // spine block summaries
+    size_t blocks_cap = 0;
     ChunkState *d_pre = nullptr;     // state before each group
     size_t pre_cap = 0;
     TileInfo *d_info = nullptr;
@@ -338,7 +340,15 @@ hipError_t ensure_scratch(Session *s, int64_t num_tiles, int mode, int table_siz
             return e;
         s->gsum_words = cap_bytes / 4;
     }
-    const size_t chunks = (size_t)((groups + chunk_groups() - 1) / chunk_groups()) + 1;
+    const size_t chunks = std::max((size_t)((groups + chunk_groups() - 1) / chunk_groups()) + 1,
+                                   (size_t)spine_blocks_needed(groups) + 1);
+    if (s->blocks_cap < chunks) {
+        size_t cap_bytes = s->blocks_cap * kMaxTablePat * sizeof(uint64_t);
+        e = grow((void **)&s->d_blocks, &cap_bytes, chunks * kMaxTablePat * sizeof(uint64_t), false, s->stream);
+        if (e != hipSuccess)
+            return e;
+        s->blocks_cap = cap_bytes / (kMaxTablePat * sizeof(uint64_t));
+    }
     if (s->chunk_cap < chunks) {
         size_t cap_bytes = s->chunk_cap * sizeof(ChunkState);
         e = grow((void **)&s->d_chunks, &cap_bytes, chunks * sizeof(ChunkState), false, s->stream);
@@ -526,6 +536,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     CK(ensure_scratch(s, num_tiles, mode, table_size));
     p.gsum = s->d_gsum;
     p.chunks = s->d_chunks;
+    p.block_tabs = s->d_blocks;
     p.chunks_pre = s->d_pre;
     p.info = s->d_info;
     p.bitmap = s->d_bitmap;
@@ -567,7 +578,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                 hipError_t e = launch_count(p, mode, table_size, gt, g0, gn, s->stream);
                 if (e != hipSuccess)
                     return e;
-                e = launch_spine(p, mode, table_size, g0, gn, c, c == num_chunks - 1, s->stream);
+                e = num_chunks == 1 ? launch_spine_blocks(p, mode, table_size, gn, s->stream)
+                                    : launch_spine(p, mode, table_size, g0, gn, c, c == num_chunks - 1, s->stream);
                 if (e != hipSuccess)
                     return e;
                 if (emit) {
@@ -646,7 +658,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                     const int64_t ng = groups_of(num_tiles, cand);
                     CK(hipEventRecord(s->ev0, s->stream));
                     CK(launch_count(p, mode, table_size, cand, 0, ng, s->stream));
-                    CK(launch_spine(p, mode, table_size, 0, ng, 0, 1, s->stream));
+                    CK(launch_spine_blocks(p, mode, table_size, ng, s->stream));
                     CK(launch_emit(p, mode, table_size, cand, 0, ng, s->stream));
                     CK(hipEventRecord(s->ev1, s->stream));
                     CK(hipEventSynchronize(s->ev1));
@@ -871,6 +883,7 @@ int session_apply_fused(Session *s)
         p.mask4 = 0xffu;
         p.gsum = s->d_gsum;
         p.chunks = s->d_chunks;
+        p.block_tabs = s->d_blocks;
         p.chunks_pre = s->d_pre;
         p.stats = s->d_stats;
         p.epoch = s->epoch;
@@ -1209,7 +1222,7 @@ void destroy(Session *s)
         if (cs)
             (void)hipStreamSynchronize(cs);
     void *dev[] = {s->d_buf, s->d_tmp, s->d_gsum, s->d_chunks, s->d_pre, s->d_info, s->d_bitmap, s->d_patrep,
-                   s->d_presence, s->d_misc, s->d_sin[0], s->d_sin[1], s->d_sout[0], s->d_sout[1]};
+                   s->d_presence, s->d_misc, s->d_sin[0], s->d_sin[1], s->d_sout[0], s->d_sout[1], s->d_blocks};
     for (void *p : dev)
         if (p)
             (void)hipFree(p);
