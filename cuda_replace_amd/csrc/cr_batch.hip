@@ -143,6 +143,32 @@ void copy_pieces(uint8_t *dst, const std::vector<Piece> &pieces, size_t total)
         t.join();
 }
 
+// Packs the documents into the pinned staging h_in (at their arena offsets) and
+// uploads them to dst, in slices of about 64 MiB: the host copy of slice k+1 runs
+// while slice k's transfer is in flight.
+hipError_t stage_and_upload(uint8_t *h_in, uint8_t *dst, const std::vector<Piece> &pieces, size_t arena_in,
+                            hipStream_t s)
+{
+    constexpr size_t kSlice = 64u << 20;
+    size_t i0 = 0;
+    while (i0 < pieces.size()) {
+        size_t i1 = i0, bytes = 0;
+        while (i1 < pieces.size() && (bytes < kSlice || i1 == i0))
+            bytes += pieces[i1++].len;
+        const std::vector<Piece> part(pieces.begin() + (long)i0, pieces.begin() + (long)i1);
+        copy_pieces(h_in, part, bytes);
+        const size_t lo = pieces[i0].dst_off;
+        const size_t hi = i1 < pieces.size() ? pieces[i1].dst_off : arena_in;
+        if (hi > lo) {
+            hipError_t e = hipMemcpyAsync(dst + lo, h_in + lo, hi - lo, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess)
+                return e;
+        }
+        i0 = i1;
+    }
+    return hipSuccess;
+}
+
 struct DocState {
     uint8_t *cur = nullptr;   // device location of the current contents
     int64_t len = 0;
@@ -295,7 +321,7 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
     std::vector<DocState> st((size_t)num_docs);
     int64_t total_pairs = 0;
     int max_pairs = 0;
-    size_t total_in = 0, arena_in = 0;
+    size_t arena_in = 0;
     std::vector<Piece> pieces;
     pieces.reserve((size_t)num_docs);
     for (int i = 0; i < num_docs; ++i) {
@@ -307,7 +333,6 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
         total_pairs += pair_counts[i];
         max_pairs = std::max(max_pairs, pair_counts[i]);
         pieces.push_back(Piece{arena_in, docs[i], doc_lens[i]});
-        total_in += doc_lens[i];
         arena_in += align_up(doc_lens[i], kDocAlign);
     }
     if (total_pairs > 0 && (!pats || !pat_lens || !reps || !rep_lens))
@@ -319,11 +344,9 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
 
     // ---- upload every document, packed at 256-byte boundaries -------------------
     CK(host_reserve(E->h_in, arena_in + 1));
-    copy_pieces(E->h_in.p, pieces, total_in);
     CK(dev_reserve(E->arena[0], arena_in + 1, false, 0, s));
     CK(dev_reserve(E->fin, align_up(arena_in + arena_in / 4, kDocAlign) + kGranule, false, 0, s));
-    if (arena_in > 0)
-        CK(hipMemcpyAsync(E->arena[0].p, E->h_in.p, arena_in, hipMemcpyHostToDevice, s));
+    CK(stage_and_upload(E->h_in.p, E->arena[0].p, pieces, arena_in, s));
     for (int i = 0; i < num_docs; ++i)
         st[(size_t)i].cur = E->arena[0].p + pieces[(size_t)i].dst_off;
 
@@ -718,7 +741,7 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     std::vector<DocState> st((size_t)num_docs);
     int64_t total_pairs = 0;
     int max_pairs = 0;
-    size_t total_in = 0, arena_in = 0, fin0 = 0;
+    size_t arena_in = 0, fin0 = 0;
     std::vector<Piece> pieces;
     pieces.reserve((size_t)num_docs);
     for (int i = 0; i < num_docs; ++i) {
@@ -729,7 +752,6 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
         total_pairs += pair_counts[i];
         max_pairs = std::max(max_pairs, pair_counts[i]);
         pieces.push_back(Piece{arena_in, docs[i], doc_lens[i]});
-        total_in += doc_lens[i];
         arena_in += align_up(doc_lens[i], kDocAlign);
         if (d.npairs == 0) {
             d.fin_off = fin0;
@@ -796,23 +818,26 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     std::vector<int64_t> rbase((size_t)max_pairs + 1, 0);
     std::vector<std::array<int32_t, kBatchClasses + 2>> rseg((size_t)max_pairs);
     {
-        int32_t *hl = reinterpret_cast<int32_t *>(E->h_rlist.p);
-        int64_t fill = 0;
+        // counting sort of (round, class, document) over every pair
+        const int nb = kBatchClasses + 1;
+        std::vector<int64_t> cnt((size_t)max_pairs * nb + 1, 0);
+        for (int i = 0; i < num_docs; ++i)
+            for (int j = 0; j < st[(size_t)i].npairs; ++j)
+                ++cnt[(size_t)j * nb + (size_t)pcls[(size_t)(st[(size_t)i].pair0 + j)] + 1];
+        for (size_t b = 1; b < cnt.size(); ++b)
+            cnt[b] += cnt[b - 1];
         for (int j = 0; j < max_pairs; ++j) {
-            rbase[(size_t)j] = fill;
-            for (int c = 0; c <= kBatchClasses; ++c) {
-                rseg[(size_t)j][(size_t)c] = (int32_t)(fill - rbase[(size_t)j]);
-                for (int i = 0; i < num_docs; ++i) {
-                    const DocState &d = st[(size_t)i];
-                    if (d.npairs > j && pcls[(size_t)(d.pair0 + j)] == c)
-                        hl[fill++] = i;
-                }
-            }
-            rseg[(size_t)j][kBatchClasses + 1] = (int32_t)(fill - rbase[(size_t)j]);
+            rbase[(size_t)j] = cnt[(size_t)j * nb];
+            for (int c = 0; c < nb; ++c)
+                rseg[(size_t)j][(size_t)c] = (int32_t)(cnt[(size_t)j * nb + (size_t)c] - rbase[(size_t)j]);
+            rseg[(size_t)j][kBatchClasses + 1] = (int32_t)(cnt[(size_t)(j + 1) * nb] - rbase[(size_t)j]);
         }
-        rbase[(size_t)max_pairs] = fill;
+        rbase[(size_t)max_pairs] = cnt[(size_t)max_pairs * nb];
+        int32_t *hl = reinterpret_cast<int32_t *>(E->h_rlist.p);
+        for (int i = 0; i < num_docs; ++i)   // documents in increasing order within each bucket
+            for (int j = 0; j < st[(size_t)i].npairs; ++j)
+                hl[cnt[(size_t)j * nb + (size_t)pcls[(size_t)(st[(size_t)i].pair0 + j)]]++] = i;
     }
-    copy_pieces(E->h_in.p, pieces, total_in);
     BatchPair *hp = reinterpret_cast<BatchPair *>(E->h_pairs.p);
     uint8_t *hpool = E->h_pairs.p + (size_t)std::max<int64_t>(total_pairs, 1) * sizeof(BatchPair);
     size_t pool_fill = 0;
@@ -841,8 +866,7 @@ int engine_run_device(Engine *E, int num_docs, const uint8_t *const *docs, const
     BatchCtl *hc = reinterpret_cast<BatchCtl *>(E->h_ctl.p);
     memset(hc, 0, sizeof(BatchCtl));
     hc->fin_used = (int64_t)fin0;
-    if (arena_in > 0)
-        CK(hipMemcpyAsync(E->arena[0].p, E->h_in.p, arena_in, hipMemcpyHostToDevice, s));
+    CK(stage_and_upload(E->h_in.p, E->arena[0].p, pieces, arena_in, s));
     if (total_pairs > 0)
         CK(hipMemcpyAsync(E->g_pairs.p, hp, (size_t)total_pairs * sizeof(BatchPair), hipMemcpyHostToDevice, s));
     if (pool_fill > 0)

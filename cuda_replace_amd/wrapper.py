@@ -68,6 +68,22 @@ def _u8(b: bytes):
     return C.cast(C.c_char_p(b), c_uint8_p)
 
 
+def _blob_arrays(items):
+    """(blob, pointer array, length array) for a list of bytes: the items joined
+    into one buffer (kept alive by the caller) and the address of each item in it."""
+    import numpy as np
+    lens = np.fromiter(map(len, items), dtype=np.int32, count=len(items))
+    blob = b"".join(items) + b"\0"
+    base = C.cast(C.c_char_p(blob), c_void_p).value
+    offs = np.zeros(max(len(items), 1), dtype=np.uint64)
+    if len(items) > 1:
+        np.cumsum(lens[:-1], dtype=np.uint64, out=offs[1:len(items)])
+    ptrs = offs + np.uint64(base)
+    if len(lens) == 0:
+        lens = np.zeros(1, dtype=np.int32)
+    return blob, ptrs, lens
+
+
 def _pair_arrays(pairs):
     """(pats, pat_lens, reps, rep_lens, keepalive) C arrays for a pair list."""
     count = len(pairs)
@@ -394,10 +410,15 @@ class BatchEngine:
         doc_ptrs = (C.c_char_p * max(n, 1))(*docs)
         doc_lens = (c_size_t * max(n, 1))(*map(len, docs))
         counts = (C.c_int * max(n, 1))(*map(len, pairs))
-        pat_ptrs = (C.c_char_p * max(m, 1))(*pats)
-        pat_lens = (C.c_int * max(m, 1))(*map(len, pats))
-        rep_ptrs = (C.c_char_p * max(m, 1))(*reps)
-        rep_lens = (C.c_int * max(m, 1))(*map(len, reps))
+        # patterns and replacements: one joined buffer each and numpy-built pointer
+        # arrays (per-element ctypes conversion of tens of thousands of pairs costs
+        # tens of milliseconds)
+        pat_blob, pat_ptr_arr, pat_len_arr = _blob_arrays(pats)
+        rep_blob, rep_ptr_arr, rep_len_arr = _blob_arrays(reps)
+        pat_ptrs = pat_ptr_arr.ctypes.data_as(C.POINTER(C.c_char_p))
+        pat_lens = pat_len_arr.ctypes.data_as(c_int_p)
+        rep_ptrs = rep_ptr_arr.ctypes.data_as(C.POINTER(C.c_char_p))
+        rep_lens = rep_len_arr.ctypes.data_as(c_int_p)
         base = c_uint8_p()
         offs = (c_size_t * max(n, 1))()
         lens = (c_size_t * max(n, 1))()
