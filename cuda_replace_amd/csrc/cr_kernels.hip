@@ -1066,14 +1066,31 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
                     }
                     if (q >= jend)
                         break;
-                    if (r < 16)
-                        store_tail(s_out + o, rep_v, r);
-                    else if (r == 16)
-                        lds_store16(s_out + o, rep_v);
-                    else
-                        lds_move(s_out + o, s_rep, r);
-                    o += r;
-                    j = q + L;
+                    if (r == 1) {
+                        // a 1-byte replacement: this and the kept starts right after it
+                        // (every L bytes) become one run of replacement bytes
+                        int k = 1;
+                        while (q + k * L < 32 && ((M >> (q + k * L)) & 1u))
+                            ++k;
+                        const uint32_t b4 = (rep_v.x & 0xffu) * 0x01010101u;
+                        const uint4 fill = {b4, b4, b4, b4};
+                        int w = 0;
+                        for (; w + 16 <= k; w += 16)
+                            lds_store16(s_out + o + w, fill);
+                        if (w < k)
+                            store_tail(s_out + o + w, fill, k - w);
+                        o += k;
+                        j = q + k * L;
+                    } else {
+                        if (r < 16)
+                            store_tail(s_out + o, rep_v, r);
+                        else if (r == 16)
+                            lds_store16(s_out + o, rep_v);
+                        else
+                            lds_move(s_out + o, s_rep, r);
+                        o += r;
+                        j = q + L;
+                    }
                     M = j >= 32 ? 0u : (M & (0xffffffffu << j));
                 }
                 __syncthreads();
