@@ -190,6 +190,12 @@ struct Session {
     bool fused_pending = false;
     std::vector<std::vector<uint8_t>> fused_pats, fused_reps;
     size_t fused_src_len = 0;
+    // the last pair list offered to apply_batch (lengths and bytes) and what the
+    // fusability check said of it: 1 fusable (h_fused and d_fused hold its table),
+    // 2 not fusable, 0 nothing cached
+    std::vector<uint8_t> fused_key;
+    int fused_key_state = 0;
+    int fused_max_grow = 0;   // largest r - L of the cached fusable list
     uint8_t *d_q[2] = {nullptr, nullptr};
     size_t cap_q[2] = {0, 0};
 
@@ -859,13 +865,16 @@ bool build_fused(const uint8_t *const *pats, const int *pat_lens, const uint8_t 
     return true;
 }
 
-// One fused pass over the session buffer with the table staged in s->h_fused.
-int session_apply_fused(Session *s)
+// One fused pass over the session buffer with the table in s->d_fused (uploaded from
+// s->h_fused first when `upload`).
+int session_apply_fused(Session *s, bool upload)
 {
     CK(ensure_misc(s));
-    if (!s->d_fused)
-        CK(hipMalloc((void **)&s->d_fused, sizeof(FusedTable)));
-    CK(hipMemcpyAsync(s->d_fused, s->h_fused, sizeof(FusedTable), hipMemcpyHostToDevice, s->stream));
+    if (upload) {
+        if (!s->d_fused)
+            CK(hipMalloc((void **)&s->d_fused, sizeof(FusedTable)));
+        CK(hipMemcpyAsync(s->d_fused, s->h_fused, sizeof(FusedTable), hipMemcpyHostToDevice, s->stream));
+    }
     const int64_t n = (int64_t)s->len;
     int64_t out = n;
     if (n > 0) {
@@ -890,19 +899,39 @@ int session_apply_fused(Session *s)
         p.write_output = 1;
         p.num_tiles = num_tiles;
         p.num_groups = num_groups;
+        // output size: at most n + (n / shortest pattern) * largest growth; when that
+        // bound is affordable (the single-pair rule) the emit follows the spine with
+        // no readback in between, otherwise the exact size is read first
+        const int64_t min_len = s->h_fused->min_len;
+        const int64_t worst = s->fused_max_grow <= 0 ? n : n + (n / min_len) * s->fused_max_grow;
+        const int64_t afford = 4 * n > (256ll << 20) ? 4 * n : (256ll << 20);
+        const bool bounded = worst <= afford;
+        auto read_total = [&](int64_t *total) -> int {
+            CK(hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s->stream));
+            CK(hipStreamSynchronize(s->stream));
+            const uint64_t kt = ((const LaunchStats *)(s->h_misc + 64))->k_total;
+            if ((uint32_t)(kt >> kEpochShift) != s->epoch)
+                return (int)hipErrorUnknown;
+            *total = n + ((int64_t)(kt << (64 - kEpochShift)) >> (64 - kEpochShift));   // sign-extend
+            return 0;
+        };
+        if (bounded)
+            CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(worst > 0 ? worst : 1), false, s->stream));
         CK(launch_fused(p, s->d_fused, 0, 0, num_groups, s->stream));
         CK(launch_fused(p, s->d_fused, 1, 0, num_groups, s->stream));
-        CK(hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s->stream));
-        CK(hipStreamSynchronize(s->stream));
-        const uint64_t kt = ((const LaunchStats *)(s->h_misc + 64))->k_total;
-        if ((uint32_t)(kt >> kEpochShift) != s->epoch)
-            return (int)hipErrorUnknown;
-        const int64_t delta = (int64_t)(kt << (64 - kEpochShift)) >> (64 - kEpochShift);   // sign-extend
-        out = n + delta;
-        CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(out > 0 ? out : 1), false, s->stream));
+        if (!bounded) {
+            int rc = read_total(&out);
+            if (rc != 0)
+                return rc;
+            CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(out > 0 ? out : 1), false, s->stream));
+        }
         p.dst = s->d_tmp;
         CK(launch_fused(p, s->d_fused, 2, 0, num_groups, s->stream));
-        CK(hipStreamSynchronize(s->stream));
+        int rc = read_total(&out);
+        if (rc != 0)
+            return rc;
+        if (bounded && out > worst)
+            return (int)hipErrorUnknown;
     } else {
         CK(grow((void **)&s->d_tmp, &s->cap_tmp, 1, false, s->stream));
     }
@@ -919,21 +948,53 @@ int session_apply_batch(Session *s, const uint8_t *const *pats, const int *pat_l
 {
     if (fused_out)
         *fused_out = 0;
-    if (allow_fuse && count >= 2) {
-        if (!s->h_fused)
-            CK(hipHostMalloc((void **)&s->h_fused, sizeof(FusedTable), hipHostMallocDefault));
-        CK(hipStreamSynchronize(s->stream));   // the staging may still feed an earlier upload
-        if (build_fused(pats, pat_lens, reps, rep_lens, count, s->h_fused)) {
+    if (allow_fuse && count >= 2 && count <= kMaxFusedPats && pats && pat_lens && reps && rep_lens) {
+        // the same pair list as the last call reuses that call's check and table
+        std::vector<uint8_t> key;
+        bool keyable = true;
+        for (int i = 0; i < count && keyable; ++i) {
+            const int L = pat_lens[i], r = rep_lens[i];
+            if (L < 0 || r < 0 || L > kMaxFusedPat || r > kMaxFusedPat || (L && !pats[i]) || (r && !reps[i])) {
+                keyable = false;
+                break;
+            }
+            key.push_back((uint8_t)L);
+            key.push_back((uint8_t)r);
+            key.insert(key.end(), pats[i], pats[i] + L);
+            key.insert(key.end(), reps[i], reps[i] + r);
+        }
+        const bool cached = keyable && s->fused_key_state != 0 && key == s->fused_key;
+        bool fusable = cached && s->fused_key_state == 1;
+        if (keyable && !cached) {
+            if (!s->h_fused)
+                CK(hipHostMalloc((void **)&s->h_fused, sizeof(FusedTable), hipHostMallocDefault));
+            CK(hipStreamSynchronize(s->stream));   // the staging may still feed an earlier upload
+            fusable = build_fused(pats, pat_lens, reps, rep_lens, count, s->h_fused);
+            s->fused_key.swap(key);
+            s->fused_key_state = fusable ? 1 : 2;
+            if (fusable) {
+                int g = INT_MIN;
+                for (int i = 0; i < count; ++i)
+                    g = std::max(g, rep_lens[i] - pat_lens[i]);
+                s->fused_max_grow = g;
+            }
+        }
+        if (fusable) {
             s->q_valid = false;
             s->fused_pending = false;
-            int rc = session_apply_fused(s);
-            if (rc != 0)
+            const bool same_pairs = cached && (int)s->fused_pats.size() == count;
+            int rc = session_apply_fused(s, !cached);
+            if (rc != 0) {
+                s->fused_key_state = 0;
                 return rc;
-            s->fused_pats.assign(count, {});
-            s->fused_reps.assign(count, {});
-            for (int i = 0; i < count; ++i) {
-                s->fused_pats[i].assign(pats[i], pats[i] + pat_lens[i]);
-                s->fused_reps[i].assign(reps[i], reps[i] + rep_lens[i]);
+            }
+            if (!same_pairs) {
+                s->fused_pats.assign(count, {});
+                s->fused_reps.assign(count, {});
+                for (int i = 0; i < count; ++i) {
+                    s->fused_pats[i].assign(pats[i], pats[i] + pat_lens[i]);
+                    s->fused_reps[i].assign(reps[i], reps[i] + rep_lens[i]);
+                }
             }
             s->fused_pending = true;
             if (fused_out)

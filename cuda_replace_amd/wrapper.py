@@ -85,16 +85,13 @@ def _blob_arrays(items):
 
 
 def _pair_arrays(pairs):
-    """(pats, pat_lens, reps, rep_lens, keepalive) C arrays for a pair list."""
-    count = len(pairs)
-    bp = [bytes(p) for p, _ in pairs]
-    br = [bytes(r) for _, r in pairs]
-    pats = (c_uint8_p * count)(*[_u8(p) for p in bp])
-    reps = (c_uint8_p * count)(*[_u8(r) for r in br])
-    pat_lens = (C.c_int * count)(*[len(p) for p in bp])
-    rep_lens = (C.c_int * count)(*[len(r) for r in br])
-    return (C.cast(pats, c_uint8_pp), C.cast(pat_lens, c_int_p), C.cast(reps, c_uint8_pp),
-            C.cast(rep_lens, c_int_p), (pats, reps, pat_lens, rep_lens, bp, br))
+    """(pats, pat_lens, reps, rep_lens, keepalive) C arrays for a pair list: the
+    patterns and the replacements each joined into one buffer, with numpy arrays of
+    their addresses and lengths (a ctypes object per pair costs ~3 us each)."""
+    pblob, pptrs, plens = _blob_arrays([bytes(p) for p, _ in pairs])
+    rblob, rptrs, rlens = _blob_arrays([bytes(r) for _, r in pairs])
+    return (pptrs.ctypes.data_as(c_uint8_pp), plens.ctypes.data_as(c_int_p), rptrs.ctypes.data_as(c_uint8_pp),
+            rlens.ctypes.data_as(c_int_p), (pblob, pptrs, plens, rblob, rptrs, rlens))
 
 
 def _load(path: str) -> C.CDLL:

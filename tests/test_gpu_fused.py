@@ -175,3 +175,33 @@ def test_fused_mixed_lengths(lib):
         pairs.append((pat, bytes(rng.choice(REP_ALPHA) for _ in range(rng.randint(1, 64)))))
     for n in (1000, 70000, 1 << 20):
         check(lib, planted(rng, n, pairs, density=0.03, boundaries=range(8192, n, 8192)), pairs)
+
+
+def test_repeated_and_changed_pair_lists(lib):
+    # a session keeps the last list's fusability check and table: a repeated list
+    # reuses them, a list differing in one byte (or one more pair, or one that is not
+    # fusable) must not
+    rng = random.Random(48)
+    pairs = entity_list(rng, 6)
+    changed = pairs[:-1] + [(pairs[-1][0], pairs[-1][1][:-1] + b"Z" if pairs[-1][1][-1:] != b"Z" else b"Y")]
+    longer = pairs + [(b"&zz;", b"W")]
+    unfusable = pairs + [(b"W", b"&a;")]
+    seq = [pairs, pairs, changed, pairs, longer, unfusable, pairs, pairs]
+    want_fused = [True, True, True, True, True, False, True, True]
+    with Session(lib, b"") as s:
+        for k, (prs, wf) in enumerate(zip(seq, want_fused)):
+            src = planted(rng, rng.choice([0, 5000, 70000]), pairs + [(b"&zz;", b"")], density=0.05)
+            s.reset(src)
+            s.apply_batch(prs)
+            assert s.last_batch_fused == wf, k
+            assert s.result() == ro.apply_batch(src, prs), k
+
+
+def test_unbounded_expansion_reads_the_size_first(lib):
+    # 2-byte patterns to 64-byte replacements over 16 MiB: the worst case (32 x the
+    # input) is beyond what a pass may allocate up front, so the exact size is read
+    # between the spine and the emit
+    rng = random.Random(49)
+    pairs = [(b"&" + bytes([c]), bytes(rng.choice(REP_ALPHA) for _ in range(64))) for c in b"abcd"]
+    src = planted(rng, 1 << 20, pairs, density=0.01) * 16
+    check(lib, src, pairs)
