@@ -463,28 +463,34 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
         if (tid == 0)
             summary[0] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
     } else {
+        // the summary is the composition of every 32-byte segment's table in position
+        // order, so each thread takes GT consecutive segments of the group and
+        // composes them itself: one wave reduction and one exchange per group.
+        // Segments at or past n have no candidates (cand_mask's range), and the
+        // carry out of the input's last group is never used.
+        const int b0 = tid * kBytesPerThread * GT;
+        const int64_t g0 = t0 * kTile;
+        Table<LT> mine = segment_table<LT>(cand_mask(p, s_in + kHalo, s_pat, b0, g0 + b0), p.L);
+#pragma unroll
+        for (int k = 1; k < GT; ++k) {
+            const int b = b0 + k * kBytesPerThread;
+            mine = tcompose<LT>(mine, segment_table<LT>(cand_mask(p, s_in + kHalo, s_pat, b, g0 + b), p.L));
+        }
+        const Table<LT> t = wave_reduce_table<LT>(mine, lane);
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < LT; ++c)
+                s_wt[wave][c] = t.e[c];
+        }
+        __syncthreads();
         Table<LT> grp = tidentity<LT>();
-        for (int i = 0; i < kGroupTiles; ++i) {
-            const int64_t ti = t0 + i;
-            if (ti >= p.num_tiles)
-                break;
-            const uint32_t m = cand_mask(p, s_in + kHalo + i * kTile, s_pat, seg, ti * kTile + seg);
-            Table<LT> t = wave_reduce_table<LT>(segment_table<LT>(m, p.L), lane);
-            if (lane == 0) {
 #pragma unroll
-                for (int c = 0; c < LT; ++c)
-                    s_wt[wave][c] = t.e[c];
-            }
-            __syncthreads();
+        for (int w = 0; w < kWaves; ++w) {
+            Table<LT> wt;
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                Table<LT> wt;
-#pragma unroll
-                for (int c = 0; c < LT; ++c)
-                    wt.e[c] = s_wt[w][c];
-                grp = tcompose<LT>(grp, wt);
-            }
-            __syncthreads();
+            for (int c = 0; c < LT; ++c)
+                wt.e[c] = s_wt[w][c];
+            grp = tcompose<LT>(grp, wt);
         }
         if (tid < LT)
             summary[tid] = tsel<LT>(grp, (uint32_t)tid);
