@@ -73,10 +73,14 @@ constexpr int kFusedPool = 16384;      // pattern + replacement bytes
 constexpr int kFusedHashBits = 10;
 constexpr int kFusedSlots = 1 << kFusedHashBits;
 
-// slot entry: key (bytes 1..key_len of the pattern, little-endian) in bits 0-31,
-// pattern index in 32-39, pattern length in 40-47, replacement length in 48-55,
-// bit 63 set; 0 = empty.  One load gives everything but the pattern's bytes.
-constexpr uint64_t kFusedUsed = 1ull << 63;
+// A slot is 16 bytes, loaded at once: x, y = the pattern's bytes 1..8 (little-endian,
+// zero past the pattern), z = meta (pattern index in bits 0-7, pattern length in
+// 8-15, replacement length in 16-23, bit 31 set; 0 = empty slot), w = the
+// replacement's first 4 bytes (zero padded).  A pattern of up to 9 bytes is verified
+// and a replacement of up to 4 bytes written from the slot alone.
+// A probe hit is the 64-bit entry meta | rep4 << 32.
+constexpr uint32_t kFusedUsed = 1u << 31;
+constexpr int kFusedSlotBytes = 8;     // pattern bytes after the first held in a slot
 
 struct FusedTable {
     int32_t count;
@@ -85,7 +89,7 @@ struct FusedTable {
     int32_t max_len;
     uint32_t key_mask;                 // low key_len bytes set
     uint32_t pad[3];
-    uint64_t slot[kFusedSlots];        // linear probing
+    uint4 slot[kFusedSlots];           // linear probing on the hash of the key bytes
     uint4 head[kMaxFusedPats];         // first 16 bytes of each pattern, zero padded
     uint4 rep_head[kMaxFusedPats];     // first 16 bytes of each replacement, zero padded
     uint16_t pat_len[kMaxFusedPats];
@@ -95,9 +99,10 @@ struct FusedTable {
     uint8_t pool[kFusedPool + 16];     // 16 bytes of slack: pattern bytes are compared 16 at a time
 };
 
-__host__ __device__ inline int fused_entry_index(uint64_t e) { return (int)((e >> 32) & 0xff); }
-__host__ __device__ inline int fused_entry_pat_len(uint64_t e) { return (int)((e >> 40) & 0xff); }
-__host__ __device__ inline int fused_entry_rep_len(uint64_t e) { return (int)((e >> 48) & 0xff); }
+__host__ __device__ inline int fused_entry_index(uint64_t e) { return (int)(e & 0xff); }
+__host__ __device__ inline int fused_entry_pat_len(uint64_t e) { return (int)((e >> 8) & 0xff); }
+__host__ __device__ inline int fused_entry_rep_len(uint64_t e) { return (int)((e >> 16) & 0xff); }
+__host__ __device__ inline uint32_t fused_entry_rep4(uint64_t e) { return (uint32_t)(e >> 32); }
 
 // Hash of the key bytes after the shared first byte (multiplicative, top bits).
 __host__ __device__ inline uint32_t fused_hash(uint32_t key)

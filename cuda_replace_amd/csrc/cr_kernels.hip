@@ -1583,37 +1583,43 @@ __device__ __forceinline__ bool head_equal(uint4 a, uint4 b, int len)
     return d == 0;
 }
 
-// Slot entry of the pattern occurring at sp (LDS; avail source bytes from there),
-// or 0.  The caller has matched the shared first byte; the key is the next key_len
-// bytes.  Two dependent table loads (slot, then the pattern's first 16 bytes) for
-// patterns of up to 16 bytes; longer ones compare the rest 16 bytes at a time.
+// Entry (meta | rep4 << 32) of the pattern occurring at sp (LDS; avail source bytes
+// from there), or 0.  The caller has matched the shared first byte; the hash is of
+// the next key_len bytes.  One table load per probe step verifies bytes 1..8 (all of
+// a pattern of up to 9 bytes); longer patterns compare the rest from their first 16
+// bytes and the pool, 16 bytes at a time.
 __device__ __forceinline__ uint64_t fused_probe(const FusedTable *t, const uint8_t *sp, int64_t avail)
 {
     if (avail < t->min_len)
         return 0;
-    uint32_t key;
-    __builtin_memcpy(&key, sp + 1, 4);
-    key &= t->key_mask;
-    const uint4 text = lds_load16(sp);
-    uint32_t h = fused_hash(key);
+    uint32_t k1, k2;   // source bytes 1..8 after the candidate
+    __builtin_memcpy(&k1, sp + 1, 4);
+    __builtin_memcpy(&k2, sp + 5, 4);
+    uint32_t h = fused_hash(k1 & t->key_mask);
     for (int probe = 0; probe < kFusedSlots; ++probe) {
-        const uint64_t e = t->slot[h];
-        if (e == 0)
+        const uint4 sl = t->slot[h];
+        if (sl.z == 0)
             return 0;
-        const int L = fused_entry_pat_len(e);
-        if ((uint32_t)e == key && L <= avail) {
-            const int k = fused_entry_index(e);
-            bool ok = head_equal(text, t->head[k], L < 16 ? L : 16);
-            if (ok && L > 16) {
-                const uint8_t *pp = t->pool + t->pat_off[k];
-                for (int i = 16; ok && i < L; i += 16) {
-                    uint4 pv;
-                    __builtin_memcpy(&pv, pp + i, 16);
-                    ok = head_equal(lds_load16(sp + i), pv, L - i < 16 ? L - i : 16);
+        const int L = (int)((sl.z >> 8) & 0xff);
+        const int nb = L - 1;   // pattern bytes after the first
+        const uint32_t mlo = nb >= 4 ? 0xffffffffu : (1u << (8 * nb)) - 1u;
+        const uint32_t mhi = nb >= 8 ? 0xffffffffu : (nb <= 4 ? 0u : (1u << (8 * (nb - 4))) - 1u);
+        if ((((k1 ^ sl.x) & mlo) | ((k2 ^ sl.y) & mhi)) == 0 && L <= avail) {
+            bool ok = true;
+            if (L > 1 + kFusedSlotBytes) {
+                const int k = (int)(sl.z & 0xff);
+                ok = head_equal(lds_load16(sp), t->head[k], L < 16 ? L : 16);
+                if (ok && L > 16) {
+                    const uint8_t *pp = t->pool + t->pat_off[k];
+                    for (int i = 16; ok && i < L; i += 16) {
+                        uint4 pv;
+                        __builtin_memcpy(&pv, pp + i, 16);
+                        ok = head_equal(lds_load16(sp + i), pv, L - i < 16 ? L - i : 16);
+                    }
                 }
             }
             if (ok)
-                return e;
+                return (uint64_t)sl.z | (uint64_t)sl.w << 32;
         }
         h = (h + 1) & (kFusedSlots - 1);
     }
@@ -1808,7 +1814,13 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
                 ++wi;
                 const int r = fused_entry_rep_len(en);
                 const int k = fused_entry_index(en);
-                if (r <= 16) {
+                if (r <= 4) {
+                    const uint4 rv = {fused_entry_rep4(en), 0u, 0u, 0u};
+                    if (staged)
+                        store_tail(s_out + o, rv, r);
+                    else
+                        store_tail(out_first + o, rv, r);
+                } else if (r <= 16) {
                     const uint4 rv = t->rep_head[k];
                     if (r == 16) {
                         if (staged)

@@ -857,10 +857,19 @@ bool build_fused(const uint8_t *const *pats, const int *pat_lens, const uint8_t 
         for (int b = 0; b < t->key_len; ++b)
             key |= (uint32_t)pats[i][1 + b] << (8 * b);
         uint32_t h = fused_hash(key);
-        while (t->slot[h] != 0)
+        while (t->slot[h].z != 0)
             h = (h + 1) & (kFusedSlots - 1);
-        t->slot[h] = (uint64_t)key | (uint64_t)i << 32 | (uint64_t)pat_lens[i] << 40 |
-                     (uint64_t)rep_lens[i] << 48 | kFusedUsed;
+        uint8_t body[kFusedSlotBytes] = {0};
+        memcpy(body, pats[i] + 1, (size_t)std::min(kFusedSlotBytes, pat_lens[i] - 1));
+        uint32_t rep4 = 0;
+        memcpy(&rep4, reps[i], (size_t)std::min(4, rep_lens[i]));
+        uint32_t lo, hi;
+        memcpy(&lo, body, 4);
+        memcpy(&hi, body + 4, 4);
+        t->slot[h].x = lo;
+        t->slot[h].y = hi;
+        t->slot[h].z = (uint32_t)i | (uint32_t)pat_lens[i] << 8 | (uint32_t)rep_lens[i] << 16 | kFusedUsed;
+        t->slot[h].w = rep4;
     }
     return true;
 }
