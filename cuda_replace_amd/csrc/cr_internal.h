@@ -135,6 +135,9 @@ struct ReplaceParams {
     ChunkState *chunks_pre; // per group: kept starts before it, carry into it
     TileInfo *info;        // per-tile records (num_tiles)
     uint32_t *bitmap;      // candidate / kept bitmap (bitmap mode)
+    uint32_t *masks;       // candidate mask of every 32-byte segment (tile * 256 + segment),
+                           // written by the count and read by the emit; null: the emit
+                           // recomputes them
     LaunchStats *stats;
     uint32_t epoch;        // 1..65535
     int32_t write_output;  // 0: count only

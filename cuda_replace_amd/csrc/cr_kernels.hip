@@ -453,8 +453,12 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
 #pragma unroll
         for (int i = 0; i < kGroupTiles; ++i) {
             const int64_t ti = t0 + i;
-            if (ti < p.num_tiles)
-                cnt += __builtin_popcount(cand_mask(p, s_in + kHalo + i * kTile, s_pat, seg, ti * kTile + seg));
+            if (ti < p.num_tiles) {
+                const uint32_t m = cand_mask(p, s_in + kHalo + i * kTile, s_pat, seg, ti * kTile + seg);
+                if (p.masks)
+                    p.masks[ti * (kTile / 32) + tid] = m;
+                cnt += __builtin_popcount(m);
+            }
         }
         cnt = wave_sum_u32(cnt);
         if (lane == 0)
@@ -470,12 +474,17 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
         // carry out of the input's last group is never used.
         const int b0 = tid * kBytesPerThread * GT;
         const int64_t g0 = t0 * kTile;
-        Table<LT> mine = segment_table<LT>(cand_mask(p, s_in + kHalo, s_pat, b0, g0 + b0), p.L);
+        // segment masks of tiles that exist are kept for the emit
+        auto seg_mask = [&](int b) -> uint32_t {
+            const uint32_t m = cand_mask(p, s_in + kHalo, s_pat, b, g0 + b);
+            if (p.masks && t0 + b / kTile < p.num_tiles)
+                p.masks[(g0 + b) / 32] = m;
+            return m;
+        };
+        Table<LT> mine = segment_table<LT>(seg_mask(b0), p.L);
 #pragma unroll
-        for (int k = 1; k < GT; ++k) {
-            const int b = b0 + k * kBytesPerThread;
-            mine = tcompose<LT>(mine, segment_table<LT>(cand_mask(p, s_in + kHalo, s_pat, b, g0 + b), p.L));
-        }
+        for (int k = 1; k < GT; ++k)
+            mine = tcompose<LT>(mine, segment_table<LT>(seg_mask(b0 + k * kBytesPerThread), p.L));
         const Table<LT> t = wave_reduce_table<LT>(mine, lane);
         if (lane == 0) {
 #pragma unroll
@@ -881,6 +890,12 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
         k_run = st.k_base;
         c_run = st.carry;
     }
+    // the count's candidate masks of my segments, loaded with the group
+    uint32_t cm[GT];
+    const bool have_masks = MODE != kModeBitmap && p.masks != nullptr;
+#pragma unroll
+    for (int i = 0; i < GT; ++i)
+        cm[i] = (have_masks && t0 + i < nt) ? p.masks[(t0 + i) * (kTile / 32) + tid] : 0u;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const uint4 rep_v = lds_load16(s_rep);   // the replacement's first 16 bytes
@@ -902,6 +917,10 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
             m = p.bitmap[ti * (kTile / 32) + tid];
             k_run = p.info[ti].k_excl;
             c_run = p.info[ti].carry_in;
+        } else if (have_masks) {
+#pragma unroll
+            for (int k = 0; k < GT; ++k)
+                m = k == i ? cm[k] : m;
         } else {
             m = cand_mask(p, s_in + in_off, s_pat, seg, start + seg);
         }

@@ -377,15 +377,20 @@ hipError_t ensure_scratch(Session *s, int64_t num_tiles, int mode, int table_siz
             return e;
         s->info_tiles = cap_bytes / sizeof(TileInfo);
     }
-    if (mode == kModeBitmap) {
-        size_t need = (size_t)num_tiles * (kTile / 32);
-        if (s->bitmap_words < need) {
-            size_t cap_bytes = s->bitmap_words * 4;
-            e = grow((void **)&s->d_bitmap, &cap_bytes, need * 4, false, s->stream);
-            if (e != hipSuccess)
+    // one word per 32-byte segment: the bitmap of a bitmap-mode pass, or the
+    // candidate masks the count hands to the emit (optional: without them the emit
+    // recomputes the masks, so a failed allocation only costs time)
+    size_t need = (size_t)num_tiles * (kTile / 32);
+    if (s->bitmap_words < need) {
+        size_t cap_bytes = s->bitmap_words * 4;
+        e = grow((void **)&s->d_bitmap, &cap_bytes, need * 4, false, s->stream);
+        if (e != hipSuccess) {
+            if (mode == kModeBitmap)
                 return e;
-            s->bitmap_words = cap_bytes / 4;
+            (void)hipGetLastError();
+            return hipSuccess;
         }
+        s->bitmap_words = cap_bytes / 4;
     }
     return hipSuccess;
 }
@@ -546,6 +551,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     p.chunks_pre = s->d_pre;
     p.info = s->d_info;
     p.bitmap = s->d_bitmap;
+    p.masks = mode != kModeBitmap && s->bitmap_words >= (size_t)num_tiles * (kTile / 32) ? s->d_bitmap : nullptr;
 
     if (mode == kModeBitmap) {
         // candidates -> serial greedy (gives the exact count) -> emit
