@@ -1670,13 +1670,18 @@ __device__ __forceinline__ void fused_count_group(const ReplaceParams &p, const 
         const int64_t gseg = ti * kTile + seg;
         const uint8_t *sp = s_in + kHalo + i * kTile + seg;
         uint32_t m = cand_mask(p, s_in + kHalo + i * kTile, s_in, seg, gseg);
+        uint32_t occ = 0;   // occurrences (all kept: they never overlap)
         while (m) {
             const int q = __builtin_ctz(m);
             m &= m - 1;
             const uint64_t e = fused_probe(t, sp + q, p.n - (gseg + q));
-            if (e)
+            if (e) {
                 delta += fused_entry_rep_len(e) - fused_entry_pat_len(e);
+                occ |= 1u << q;
+            }
         }
+        if (p.masks)
+            p.masks[ti * (kTile / 32) + tid] = occ;
     }
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1)
@@ -1723,6 +1728,12 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
     load_group<kGroupTiles>(p, t0, tid, true, s_in);
     int64_t k_run = pre->k_base;   // length change before the current tile
     int c_run = 0;                 // leading bytes of the current tile covered by an earlier match
+    // the count's occurrence masks of my segments, loaded with the group
+    uint32_t cm[kGroupTiles];
+    const bool have_masks = p.masks != nullptr;
+#pragma unroll
+    for (int i = 0; i < kGroupTiles; ++i)
+        cm[i] = (have_masks && t0 + i < nt) ? p.masks[(t0 + i) * (kTile / 32) + tid] : 0u;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const int seg = tid * kBytesPerThread;
@@ -1738,8 +1749,16 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
         const int jend = seg_valid < 32 ? (seg_valid > 0 ? seg_valid : 0) : 32;
         const int in_off = kHalo + i * kTile;
         const uint8_t *sp = s_in + in_off + seg;
-        // my occurrences: kept mask, summed length change, end of the last one
-        uint32_t m = cand_mask(p, s_in + in_off, s_in, seg, start + seg);
+        // my occurrences: kept mask, summed length change, end of the last one (the
+        // count's mask holds only occurrences; without it, every candidate is probed)
+        uint32_t m = 0;
+        if (have_masks) {
+#pragma unroll
+            for (int k = 0; k < kGroupTiles; ++k)
+                m = k == i ? cm[k] : m;
+        } else {
+            m = cand_mask(p, s_in + in_off, s_in, seg, start + seg);
+        }
         uint32_t kept = 0;
         int d = 0;
         int e = kNegInf;

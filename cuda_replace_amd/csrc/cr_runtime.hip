@@ -909,6 +909,7 @@ int session_apply_fused(Session *s, bool upload)
         p.chunks = s->d_chunks;
         p.block_tabs = s->d_blocks;
         p.chunks_pre = s->d_pre;
+        p.masks = s->bitmap_words >= (size_t)num_tiles * (kTile / 32) ? s->d_bitmap : nullptr;
         p.stats = s->d_stats;
         p.epoch = s->epoch;
         p.write_output = 1;
