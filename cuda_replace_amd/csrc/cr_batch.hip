@@ -17,6 +17,7 @@
 // and pair.
 #include "cr_internal.h"
 #include "cr_pattern.h"
+#include "cr_host.h"
 #include "../../include/cuda_replace.h"
 
 #include <algorithm>
@@ -119,8 +120,7 @@ struct Piece {
 // Host memcpy of many pieces into one buffer, split over threads by bytes.
 void copy_pieces(uint8_t *dst, const std::vector<Piece> &pieces, size_t total)
 {
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int nthreads = total < (32u << 20) ? 1 : (int)std::min<unsigned>(hw ? hw : 1, 8);
+    const int nthreads = total < (32u << 20) ? 1 : host_copy_threads();
     auto run = [&](size_t lo, size_t hi) {   // byte range [lo, hi) of the concatenation
         size_t acc = 0;
         for (const Piece &pc : pieces) {
