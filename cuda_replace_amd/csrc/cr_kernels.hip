@@ -532,14 +532,22 @@ struct SpineShape {
     static constexpr int V = W == 1 ? 16 : (16 / W > 1 ? 16 / W : 1);   // groups per lane and pass
 };
 
+// Groups per lane and step of a blocked spine: carry tables take one group per lane
+// (a block is a single short table step per wave, so a pass spreads over many
+// workgroups: 2-state all-'a' pass 0.774 -> 0.766 ms); kept counts keep 16 per lane
+// (1024-group blocks measured no faster there).
 template <int MODE, int LT>
-constexpr int64_t spine_pass_groups()
+constexpr int spine_block_v()
 {
-    constexpr int W = MODE == kModeTable ? LT : 1;
-    return (int64_t)kWave * SpineShape<W>::V * kSpineWaves;
+    return MODE == kModeTable ? 1 : SpineShape<1>::V;
+}
+template <int MODE, int LT>
+constexpr int64_t spine_block_groups()
+{
+    return (int64_t)kWave * spine_block_v<MODE, LT>() * kSpineWaves;
 }
 
-// BLOCKS: one workgroup per block of spine_pass_groups() groups (block = chunk =
+// BLOCKS: one workgroup per block of spine_block_groups() groups (block = chunk =
 // blockIdx.x, its entering state from chunks[]), the pass total published by the
 // block scan instead.
 template <int MODE, int LT, bool BLOCKS = false>
@@ -547,7 +555,7 @@ __global__ void __launch_bounds__(kSpineThreads)
 spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk, int last_chunk)
 {
     constexpr int W = MODE == kModeTable ? LT : 1;   // words per group summary
-    constexpr int V = SpineShape<W>::V;
+    constexpr int V = BLOCKS ? spine_block_v<MODE, LT>() : SpineShape<W>::V;
     constexpr int64_t kBlock = (int64_t)kWave * V;   // groups per wave and pass
     constexpr int64_t kPass = kBlock * kSpineWaves;
     __shared__ uint64_t s_wt[kSpineWaves][LT];
@@ -751,7 +759,7 @@ spine_kernel(ReplaceParams p, int64_t first_group, int64_t groups, int64_t chunk
 template <int MODE, int LT>
 __global__ void __launch_bounds__(kSpineThreads) spine_reduce_kernel(ReplaceParams p, int64_t groups)
 {
-    constexpr int64_t kPass = spine_pass_groups<MODE, LT>();
+    constexpr int64_t kPass = spine_block_groups<MODE, LT>();
     __shared__ uint64_t s_wt[kSpineWaves][LT];
     __shared__ int64_t s_wsum[kSpineWaves];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -2258,7 +2266,7 @@ hipError_t launch_count(const ReplaceParams &p, int mode, int table_size, int gt
 template <int MODE, int LT>
 hipError_t spine_blocks_t(const ReplaceParams &p, int64_t groups, hipStream_t s)
 {
-    constexpr int64_t kPass = spine_pass_groups<MODE, LT>();
+    constexpr int64_t kPass = spine_block_groups<MODE, LT>();
     const int64_t nb = (groups + kPass - 1) / kPass;
     if (nb <= 1)   // one block: the single-workgroup spine is the whole job
         return spine_t<MODE, LT>(p, 0, groups, 0, 1, s);
@@ -2271,7 +2279,7 @@ hipError_t spine_blocks_t(const ReplaceParams &p, int64_t groups, hipStream_t s)
 
 int64_t spine_blocks_needed(int64_t groups)
 {
-    // the smallest block (carry tables of 16 states) has 64 x 1 x 16 groups
+    // the smallest block has 64 x 1 x 16 groups
     return groups / ((int64_t)kWave * kSpineWaves) + 2;
 }
 

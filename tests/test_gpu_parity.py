@@ -280,3 +280,23 @@ def test_all_a_64mib_closed_form(lib):
         s.reset(b"a" * (n + 1))
         assert s.apply(b"aa", b"X") == n // 2 + 1
         assert s.result() == b"X" * (n // 2) + b"a"
+
+
+# ------------------------------------------------------------ count first, then emit
+@pytest.mark.parametrize("pat", [b"a", b"aa", b"aba"])
+def test_expansion_beyond_the_upfront_bound(lib, pat):
+    # 16 MiB with a 64-byte replacement: the worst case (up to 64 x the input) is
+    # beyond what a pass allocates up front, so count and spine run first, the
+    # exact size is read back, and the emit runs later over the candidate masks
+    # the count left behind; once plain and once seeded
+    from cuda_replace_amd import Session
+    rng = random.Random(131)
+    unit = bytes(rng.choice(b"abcdefgh") for _ in range(1 << 16))
+    src = unit * 256
+    rep = bytes(rng.choice(b"XYZ") for _ in range(64))
+    assert lib.unified(src, pat, rep) == src.replace(pat, rep)
+    with Session(lib, src) as s:
+        seed = 5_000_003
+        n2, last = s.apply_seeded(pat, rep, seed)
+        want, want_last = seeded_contract(src, pat, rep, seed)
+        assert s.result() == want and n2 == len(want) and last == want_last

@@ -2,9 +2,9 @@
 scan of the block summaries, then every block's groups in parallel): the kept
 count and the carry must cross block boundaries exactly, for border-free
 patterns, carry tables of 2/4/8/16 states, seeded applies and the fused
-multi-pattern pass.  A block holds 16384 groups (border-free, fused), 8192 / 4096
-/ 2048 / 1024 (carry tables of 2 / 4 / 8 / 16 states); group size pinned to one
-tile (8 KiB) where that keeps the inputs small."""
+multi-pattern pass.  A block holds 16384 groups (border-free, fused) or 1024
+(carry tables); group size pinned to one tile (8 KiB) where that keeps the inputs
+small, and a pass of exactly one block plus one group."""
 import random
 
 import pytest
@@ -52,7 +52,7 @@ def test_border_free_three_blocks(lib, one_tile_groups):
 @pytest.mark.parametrize("pat,size", [(b"aa", 140 * MiB), (b"aba", 70 * MiB), (b"aabaa", 40 * MiB),
                                       (b"a" * 12, 20 * MiB)])
 def test_carry_tables_across_blocks(lib, one_tile_groups, pat, size):
-    lib.autotune_set(pat, size, 1)     # 17920 / 8960 / 5120 / 2560 one-tile groups: 3 blocks each
+    lib.autotune_set(pat, size, 1)     # 17920 / 8960 / 5120 / 2560 one-tile groups: 3-18 blocks
     for src in (b"a" * size, _text(size, len(pat))):
         assert lib.unified(src, pat, b"Q") == src.replace(pat, b"Q"), (pat, size)
 
@@ -70,3 +70,13 @@ def test_fused_pass_across_blocks(lib):
         s.apply_batch(pairs)
         assert s.last_batch_fused
         assert s.result() == ro.apply_batch(src, pairs)
+
+
+@pytest.mark.parametrize("pat,rep,blk", [(b"ab", b"XYZ", 16384), (b"aa", b"Q", 1024), (b"aabaa", b"", 1024)])
+def test_one_block_and_one_group(lib, one_tile_groups, pat, rep, blk):
+    # blk + 1 one-tile groups: a full spine block and a block of a single group (its
+    # only tile a partial one), for border-free and carry-table patterns
+    n = blk * 8192 + 100
+    src = _text(n, 9)
+    lib.autotune_set(pat, n, 1)
+    assert lib.unified(src, pat, rep) == src.replace(pat, rep)
