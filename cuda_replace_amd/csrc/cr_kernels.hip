@@ -1915,7 +1915,9 @@ fused_count_kernel(ReplaceParams p, const FusedTable *t, int64_t first_group)
     fused_count_group(p, t, g, reinterpret_cast<int32_t *>(p.gsum) + g);
 }
 
-__global__ void __launch_bounds__(kThreads)
+// Six waves per SIMD, as many as its LDS allows: left alone the compiler takes 91
+// VGPRs, which leaves room for five.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6)))
 fused_emit_kernel(ReplaceParams p, const FusedTable *t, int64_t first_group)
 {
     const int64_t g = first_group + blockIdx.x;
