@@ -86,14 +86,33 @@ struct Table {
     uint32_t e[LT];
 };
 
+// Entry idx (< LT) of a table: a binary tree of selects on the index bits, level
+// by level.  Written as a chain of equality selects, the compiler turns the lookup
+// into an indexed array in scratch memory for LT >= 4.
+template <int LT, class T>
+__device__ __forceinline__ T tree_select(const T (&e)[LT], uint32_t idx)
+{
+    T v[LT];
+#pragma unroll
+    for (int i = 0; i < LT; ++i)
+        v[i] = e[i];
+    constexpr int kLevels = LT >= 16 ? 4 : LT >= 8 ? 3 : LT >= 4 ? 2 : LT >= 2 ? 1 : 0;
+#pragma unroll
+    for (int b = 0; b < kLevels; ++b) {
+        // a bitwise select: a select of two table entries is also turned into an
+        // indexed load from a stack copy of the table
+        const T m = (T)0 - (T)((idx >> b) & 1u);
+#pragma unroll
+        for (int i = 0; i < (LT >> (b + 1)); ++i)
+            v[i] = v[2 * i] ^ ((v[2 * i] ^ v[2 * i + 1]) & m);
+    }
+    return v[0];
+}
+
 template <int LT>
 __device__ __forceinline__ uint32_t tsel(const Table<LT> &t, uint32_t idx)
 {
-    uint32_t v = t.e[0];
-#pragma unroll
-    for (int i = 1; i < LT; ++i)
-        v = (idx == (uint32_t)i) ? t.e[i] : v;
-    return v;
+    return tree_select<LT, uint32_t>(t.e, idx);
 }
 
 // apply `first` then `second`
@@ -212,11 +231,7 @@ struct Table64 {
 template <int LT>
 __device__ __forceinline__ uint64_t tsel64(const Table64<LT> &t, uint32_t idx)
 {
-    uint64_t v = t.e[0];
-#pragma unroll
-    for (int i = 1; i < LT; ++i)
-        v = (idx == (uint32_t)i) ? t.e[i] : v;
-    return v;
+    return tree_select<LT, uint64_t>(t.e, idx);
 }
 
 // apply `first` then `second`
