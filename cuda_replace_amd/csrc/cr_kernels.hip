@@ -834,32 +834,54 @@ __global__ void __launch_bounds__(kSpineThreads) spine_reduce_kernel(ReplacePara
     }
 }
 
-// Spine blocks, second launch (one thread): the state entering every block into
-// chunks[b], the state after the last into chunks[nblocks], and the pass total.
+// Spine blocks, second launch (one wave, one walking thread): the state entering
+// every block into chunks[b], the state after the last into chunks[nblocks], and
+// the pass total.
 template <int MODE, int LT>
-__global__ void spine_top_kernel(ReplaceParams p, int64_t nblocks)
+__global__ void __launch_bounds__(kWave) spine_top_kernel(ReplaceParams p, int64_t nblocks)
 {
-    if (threadIdx.x != 0)
-        return;
+    // the walk is serial (each block's entering carry selects its table entry), so
+    // the wave first stages 64 blocks' summaries in LDS with independent loads and
+    // lane 0 walks them there, instead of one dependent HBM round trip per block
+    constexpr int W = MODE == kModeTable ? LT : 1;
+    __shared__ uint64_t s_tab[kWave * W];
+    const int lane = threadIdx.x;
     int64_t k = 0;
     uint32_t carry = 0;
-    for (int64_t b = 0; b <= nblocks; ++b) {
+    for (int64_t b0 = 0; b0 < nblocks; b0 += kWave) {
+        const int cnt = nblocks - b0 < kWave ? (int)(nblocks - b0) : kWave;
+        if (lane < cnt) {
+#pragma unroll
+            for (int c = 0; c < W; ++c)
+                s_tab[lane * W + c] = p.block_tabs[(b0 + lane) * kMaxTablePat + c];
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int i = 0; i < cnt; ++i) {
+                ChunkState cs;
+                cs.k_base = k;
+                cs.carry = (int)carry;
+                cs.pad = 0;
+                p.chunks[b0 + i] = cs;
+                if constexpr (MODE != kModeTable) {
+                    k += (int64_t)s_tab[i];
+                } else {
+                    const uint64_t e = s_tab[i * W + carry];
+                    k += (int64_t)(e & kCountMask);
+                    carry = (uint32_t)(e >> kCarryShift);
+                }
+            }
+        }
+        __syncthreads();   // s_tab is refilled for the next 64 blocks
+    }
+    if (lane == 0) {
         ChunkState cs;
         cs.k_base = k;
         cs.carry = (int)carry;
         cs.pad = 0;
-        p.chunks[b] = cs;
-        if (b == nblocks)
-            break;
-        if constexpr (MODE != kModeTable) {
-            k += (int64_t)p.block_tabs[b * kMaxTablePat];
-        } else {
-            const uint64_t e = p.block_tabs[b * kMaxTablePat + carry];
-            k += (int64_t)(e & kCountMask);
-            carry = (uint32_t)(e >> kCarryShift);
-        }
+        p.chunks[nblocks] = cs;
+        p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | ((uint64_t)k & ((1ull << kEpochShift) - 1));
     }
-    p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | ((uint64_t)k & ((1ull << kEpochShift) - 1));
 }
 
 // ---- emit: resolve kept starts and write each tile's output ----------------
