@@ -1200,8 +1200,16 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
     }
 }
 
+// Waves per SIMD the emit is compiled for (1: the compiler's choice).  The 8- and
+// 16-state emits are held to 96 VGPRs so that five fit instead of four.
 template <int MODE, int LT, int GT>
-__global__ void __launch_bounds__(kThreads)
+constexpr int emit_waves_per_eu()
+{
+    return MODE == kModeTable && LT >= 8 && GT <= 2 ? 5 : 1;
+}
+
+template <int MODE, int LT, int GT>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(emit_waves_per_eu<MODE, LT, GT>())))
 emit_kernel(ReplaceParams p, int64_t first_group)
 {
     const int64_t g = first_group + blockIdx.x;
