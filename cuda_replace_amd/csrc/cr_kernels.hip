@@ -1136,7 +1136,12 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
                     }
                     if (q >= jend)
                         break;
-                    if (r == 1) {
+                    if (r == 1 && !(q + L < 32 && ((M >> (q + L)) & 1u))) {
+                        // a 1-byte replacement with no kept start right after it
+                        s_out[o] = (uint8_t)rep_v.x;
+                        o += 1;
+                        j = q + L;
+                    } else if (r == 1) {
                         // a 1-byte replacement: this and the kept starts right after it
                         // (every L bytes) become one run of replacement bytes
                         int k = 1;
