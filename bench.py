@@ -388,8 +388,8 @@ def main() -> int:
     ap.add_argument("--session-gib", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the replace path runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one GPU: the replace path runs on a single MI355X; "
+              "sharding over several GPUs is not built", file=sys.stderr)
         return 2
 
     from cuda_replace_amd import CudaReplaceLib, workloads
