@@ -300,3 +300,23 @@ def test_expansion_beyond_the_upfront_bound(lib, pat):
         n2, last = s.apply_seeded(pat, rep, seed)
         want, want_last = seeded_contract(src, pat, rep, seed)
         assert s.result() == want and n2 == len(want) and last == want_last
+
+
+def test_modes_alternate_on_one_buffer(lib):
+    # passes of every mode applied one after another to the same buffer (no reset):
+    # the bitmap mode's bitmap and the count's candidate masks share one scratch
+    # buffer, and each pass must see its own; a prefix query after each pass
+    from cuda_replace_amd import Session
+    rng = random.Random(141)
+    src = bytes(rng.choice(b"ab \n") for _ in range(3 << 20))
+    steps = [(b"ab", b"X"), (b"ab" * 9 + b"a", b"Q"), (b"aa", b"a"), (b"X", b"ba"), (b"b" * 20, b""),
+             (b"aba", b"Z"), (b" \n", b"ab")]
+    want = src
+    with Session(lib, src) as s:
+        for pat, rep in steps:
+            s.apply(pat, rep)
+            prev = want
+            want = want.replace(pat, rep)
+            assert s.result() == want, (pat, lib.pattern_mode(pat))
+            lim = len(prev) // 2
+            assert s.query_prefix(lim) == ro.query_prefix(prev, pat, None, lim), pat
