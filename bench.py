@@ -14,7 +14,12 @@ roofline = algorithmic bytes of a border-free pass (n_in + n_out) over the pass'
            stream, against 8 TB/s; traffic (HBM bytes per pass, counter-calibrated)
            comes from the committed rocprofv3 summary (profiles/traffic.json).
 cpu_baseline = bytes.replace, single thread, on the same three inputs (one pass
-           each, best of two), plus a fork-based per-core chunked variant.
+           each, best of two), plus a per-core chunked variant: one forked process
+           per safe-boundary chunk (16 host cores, the box's share for one GPU),
+           started together, each returning its piece's length and digest.
+c1       = the two 64 MiB configurations (seed-0 uniform {a,b,c,d} with ab -> X,
+           all-'a' with aa -> X), each timed cache-resident and with the memory-side
+           cache flushed between passes, each output checked byte for byte.
 batched  = the uneven-documents workload (8192 documents, log-normal sizes, 1-16
            pairs each) through the batch engine (device-driven rounds replayed as a
            HIP graph), host to host, every result checked against the oracle; the
@@ -26,7 +31,10 @@ serving  = open-loop Poisson arrivals (1000 requests/s for --serve-seconds) of
            requests sliced from the text (log-normal sizes, median 1 MiB, sigma 2,
            4 KiB - 256 MiB; 1-32 pairs each) into the continuous-batching server
            (4 workers, 200 GB budget): p50 / p99 latency from scheduled arrival to
-           completion, aggregate input GB/s, a sample checked against the oracle.
+           completion, aggregate input GB/s, a sample checked against the oracle
+           after the timed section; beside it the server's capacity on a burst of
+           the same mix and the per-call loop (one cuda_replace_unified per pair) on
+           the same mix, and their ratio.
 streaming = the text repeated to --stream-gib GiB, streamed through the pipelined
            chunked path in 256 MiB chunks (fragment -> 12 bytes), host to host,
            checked against bytes.replace of the 1 GiB piece.
@@ -67,6 +75,79 @@ def log(*a):
 _CPU_DATA = {}
 
 
+def _digest(b) -> bytes:
+    """128-bit digest of a result piece (xxh3 when installed, else blake2b)."""
+    try:
+        import xxhash
+        return xxhash.xxh3_128_digest(b)
+    except ImportError:
+        import hashlib
+        return hashlib.blake2b(b, digest_size=16).digest()
+
+
+def _fork_map(fn, tasks, procs: int):
+    """fn over tasks in `procs` forked children (task i runs in child i % procs).
+
+    The children are forked first and block on a shared pipe; closing its write
+    end starts them all at once, so the returned wall time covers the work and the
+    return of the (small) results, not the forks.  Every child is reaped with
+    waitpid before this returns.  Called only before the GPU is touched.
+    Returns (results in task order, wall seconds)."""
+    import pickle
+    procs = max(1, min(procs, len(tasks)))
+    go_r, go_w = os.pipe()
+    kids = []
+    try:
+        for w in range(procs):
+            r, wr = os.pipe()
+            pid = os.fork()
+            if pid == 0:                                  # child
+                code = 1
+                try:
+                    os.close(go_w)
+                    os.close(r)
+                    os.read(go_r, 1)                      # EOF once the parent closes go_w
+                    res = [(i, fn(tasks[i])) for i in range(w, len(tasks), procs)]
+                    data = pickle.dumps(res, protocol=pickle.HIGHEST_PROTOCOL)
+                    with os.fdopen(wr, "wb") as f:
+                        f.write(data)
+                    code = 0
+                finally:
+                    os._exit(code)
+            os.close(wr)
+            kids.append((pid, r))
+        t0 = time.perf_counter()
+        os.close(go_w)
+        go_w = -1
+        out = [None] * len(tasks)
+        failed = False
+        for pid, r in kids:
+            with os.fdopen(r, "rb") as f:
+                data = f.read()
+            _, status = os.waitpid(pid, 0)
+            pid = 0
+            if status != 0 or not data:
+                failed = True
+                continue
+            for i, v in pickle.loads(data):
+                out[i] = v
+        wall = time.perf_counter() - t0
+        kids = []
+    finally:
+        if go_w >= 0:
+            os.close(go_w)
+        os.close(go_r)
+        for pid, r in kids:                               # only on an error path
+            try:
+                os.kill(pid, 9)
+                os.waitpid(pid, 0)
+            except OSError:
+                pass
+    if failed:
+        raise RuntimeError("a CPU worker failed")
+    return out, wall
+
+
 def _safe_cuts(data: bytes, pat: bytes, parts: int):
     """Cut positions where no occurrence of pat starts in (b-L, b): the greedy
     state resets there, so per-piece bytes.replace concatenates exactly."""
@@ -89,20 +170,40 @@ def _safe_cuts(data: bytes, pat: bytes, parts: int):
 
 
 def _cpu_piece(args):
+    """One chunk of the per-core CPU baseline: bytes.replace of the slice, timed on
+    its own; returns (output length, digest, replace seconds)."""
     key, a, b, pat, rep = args
-    return _CPU_DATA[key][a:b].replace(pat, rep)
+    t = time.perf_counter()
+    out = _CPU_DATA[key][a:b].replace(pat, rep)
+    dt = time.perf_counter() - t
+    return len(out), _digest(out), dt
 
 
-def cpu_parallel(key: str, pat: bytes, rep: bytes, procs: int):
+def cpu_parallel(key: str, pat: bytes, rep: bytes, procs: int, expected: bytes):
+    """Per-core chunked bytes.replace over safe cuts, one chunk per process; the
+    pieces are checked against the single-thread output by digest.
+    Returns (wall s, pieces, slowest worker's replace s, exact)."""
     data = _CPU_DATA[key]
     cuts = _safe_cuts(data, pat, procs)
-    ctx = mp.get_context("fork")
-    with ctx.Pool(procs) as pool:
-        t = time.perf_counter()
-        pieces = pool.map(_cpu_piece, [(key, cuts[i], cuts[i + 1], pat, rep) for i in range(len(cuts) - 1)])
-        out = b"".join(pieces)
-        dt = time.perf_counter() - t
-    return out, dt, len(cuts) - 1
+    tasks = [(key, cuts[i], cuts[i + 1], pat, rep) for i in range(len(cuts) - 1)]
+    res, wall = _fork_map(_cpu_piece, tasks, procs)
+    off, exact = 0, True
+    for ln, dg, _ in res:
+        exact &= _digest(expected[off:off + ln]) == dg
+        off += ln
+    exact &= off == len(expected)
+    return wall, len(tasks), max(r[2] for r in res), exact
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 _SB = {}
@@ -111,14 +212,16 @@ _SB = {}
 def _sb_piece(args):
     a, b = args
     from oracle import replace_oracle as ro
-    return ro.apply_batch(_SB["src"][a:b], _SB["pairs"])
+    out = ro.apply_batch(_SB["src"][a:b], _SB["pairs"])
+    return len(out), _digest(out)
 
 
 def session_batch_prepare(ro, gib: float, procs: int) -> dict:
-    """Input, expected output and single-thread CPU sample of the session batch.
-    Runs before the GPU is initialised (it forks).  Slices are cut just after a
-    space: no pattern holds a space and no pass removes one, so every pass's
-    occurrences stay inside a slice and the slices' results concatenate exactly."""
+    """Input, expected output digests and single-thread CPU sample of the session
+    batch.  Runs before the GPU is initialised (it forks).  Slices are cut just after
+    a space: no pattern holds a space and no pass removes one, so every pass's
+    occurrences stay inside a slice and the slices' results concatenate exactly;
+    each slice's expected output is kept as (length, digest)."""
     from cuda_replace_amd import workloads
     pairs = workloads.entity_pairs()
     n = int(gib * (1 << 30))
@@ -135,23 +238,30 @@ def session_batch_prepare(ro, gib: float, procs: int) -> dict:
         if c != -1 and c + 1 > cuts[-1]:
             cuts.append(c + 1)
     cuts.append(n)
-    t = time.perf_counter()
-    with mp.get_context("fork").Pool(procs) as pool:
-        expected = b"".join(pool.map(_sb_piece, [(cuts[i], cuts[i + 1]) for i in range(len(cuts) - 1)]))
-    log(f"[bench] session batch oracle: {time.perf_counter() - t:.1f}s on {procs} processes")
+    slices, wall = _fork_map(_sb_piece, [(cuts[i], cuts[i + 1]) for i in range(len(cuts) - 1)], procs)
+    log(f"[bench] session batch oracle: {wall:.1f}s on {procs} processes")
     sample_n = src.find(b" ", min(n, 64 << 20)) + 1 or n
     t = time.perf_counter()
     ro.apply_batch(src[:sample_n], pairs)
     cpu_s = time.perf_counter() - t
     _SB.clear()
-    return {"src": src, "pairs": pairs, "expected": expected, "cpu_sample_bytes": sample_n, "cpu_s": cpu_s,
-            "oracle_procs": procs}
+    return {"src": src, "pairs": pairs, "expected": slices, "expected_len": sum(ln for ln, _ in slices),
+            "cpu_sample_bytes": sample_n, "cpu_s": cpu_s, "oracle_procs": procs}
+
+
+def _matches_slices(out: bytes, slices) -> bool:
+    off = 0
+    for ln, dg in slices:
+        if _digest(out[off:off + ln]) != dg:
+            return False
+        off += ln
+    return off == len(out)
 
 
 def session_batch_bench(lib, prep: dict, steps: int) -> dict:
     """Fused vs pair-by-pair cuda_replace_apply_batch on one resident session."""
     from cuda_replace_amd import Session
-    src, pairs, expected = prep["src"], prep["pairs"], prep["expected"]
+    src, pairs, slices, exp_len = prep["src"], prep["pairs"], prep["expected"], prep["expected_len"]
     n = len(src)
     fusable = lib.batch_fusable(pairs)
     with Session(lib, src) as s:
@@ -163,21 +273,21 @@ def session_batch_bench(lib, prep: dict, steps: int) -> dict:
             out_len = s.apply_batch(pairs)
             fused_t.append(time.perf_counter() - t)
         fused = s.last_batch_fused
-        exact_fused = out_len == len(expected) and s.result() == expected
+        exact_fused = out_len == exp_len and _matches_slices(s.result(), slices)
         s.reset(src)
         t = time.perf_counter()
         s.apply_batch(pairs, fuse=False)
         seq_t = time.perf_counter() - t
-        exact_seq = s.result() == expected
+        exact_seq = _matches_slices(s.result(), slices)
     f_ms = statistics.median(fused_t) * 1e3
     log(f"[bench] session batch: fused={fused} {f_ms:.2f} ms, pair by pair {seq_t * 1e3:.1f} ms, "
         f"bit_exact fused={exact_fused} sequential={exact_seq}")
-    algo = n + len(expected)
+    algo = n + exp_len
     return {
         "workload": f"{n / 2**30:.0f} GiB resident text, 128 pairs b'&'+3-6 letters+b';' -> 1-4 B of A-Z0-9 in "
                     f"order, patterns at 0.5% of positions, NUL at 0.01%",
         "input_bytes": n,
-        "output_bytes": len(expected),
+        "output_bytes": exp_len,
         "fusable": fusable,
         "fused": fused,
         "fused_ms": round(f_ms, 3),
@@ -194,6 +304,79 @@ def session_batch_bench(lib, prep: dict, steps: int) -> dict:
                                    f"128 bytes.replace passes, single thread"},
         "bit_exact": bool(exact_fused and exact_seq),
     }
+
+
+# ------------------------------------------------------------------ 64 MiB configurations
+C1_N = 64 << 20
+
+
+def c1_prepare(ro) -> dict:
+    """The two 64 MiB inputs (seed-0 uniform {a,b,c,d}, ab -> X; all-'a', aa -> X),
+    their bytes.replace outputs and the single-thread CPU time (best of 3)."""
+    from cuda_replace_amd import workloads
+    cases = [("uniform_abcd_ab_X", workloads.uniform_abcd(C1_N, 0), b"ab", b"X"),
+             ("all_a_aa_X", workloads.all_a(C1_N), b"aa", b"X")]
+    out = []
+    for name, src, pat, rep in cases:
+        best = None
+        for _ in range(3):
+            t = time.perf_counter()
+            exp = ro.replace(src, pat, rep)
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        out.append((name, src, pat, rep, exp, best))
+        log(f"[bench] c1 cpu {name}: {best * 1e3:.1f} ms ({C1_N / best / 1e9:.2f} GB/s)")
+    return {"cases": out}
+
+
+def c1_bench(lib, h, prep: dict, dev, runs: int = 20) -> dict:
+    """Each 64 MiB pass timed with HIP events on the library's stream: back to back
+    (the input stays in the 256 MB memory-side cache: "cache_resident") and with a
+    1 GiB buffer written between passes ("flushed"); median of `runs` each; the
+    output checked byte for byte."""
+    import torch
+    flush = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    out_len = C.c_size_t()
+    kms = C.c_float()
+    res = {"workload": "64 MiB seed-0 uniform {a,b,c,d} with ab->X; 64 MiB all-'a' with aa->X; device-resident",
+           "timing": "HIP events around the pass's launches on the library stream, median of %d" % runs}
+    for name, src, pat, rep, exp, cpu_s in prep["cases"]:
+        d_src = torch.frombuffer(bytearray(src), dtype=torch.uint8).to(dev)
+        d_dst = torch.empty(len(src) + 16, dtype=torch.uint8, device=dev)
+
+        def once():
+            rc = lib.lib.cr_replace_device(h, C.c_void_p(d_src.data_ptr()), len(src), C.c_void_p(d_dst.data_ptr()),
+                                           d_dst.numel(), C.cast(C.c_char_p(pat), C.POINTER(C.c_uint8)), len(pat),
+                                           C.cast(C.c_char_p(rep), C.POINTER(C.c_uint8)), len(rep), -len(pat),
+                                           C.byref(out_len), None, C.byref(kms))
+            if rc != 0:
+                raise RuntimeError(f"cr_replace_device({name}) rc={rc}")
+            return kms.value
+
+        for _ in range(3):
+            once()
+        cached = [once() for _ in range(runs)]
+        flushed = []
+        for k in range(runs):
+            flush.fill_(k & 0xff)
+            torch.cuda.synchronize()
+            flushed.append(once())
+        ol = out_len.value
+        exact = ol == len(exp) and d_dst[:ol].cpu().numpy().tobytes() == exp
+        algo = len(src) + ol
+        entry = {"out_len": ol, "bit_exact": bool(exact), "pattern_mode": lib.pattern_mode(pat),
+                 "cpu_single_thread_GBps": round(len(src) / cpu_s / 1e9, 3)}
+        for label, ts in (("cache_resident", cached), ("flushed", flushed)):
+            med = statistics.median(ts)
+            entry[label] = {"kernel_us_median": round(med * 1e3, 1), "input_GBps": round(len(src) / med / 1e6, 1),
+                            "algo_GBps": round(algo / med / 1e6, 1), "hbm_frac": round(algo / (med / 1e3) / HBM_PEAK, 4)}
+        res[name] = entry
+        log(f"[bench] c1 {name}: cached {entry['cache_resident']['kernel_us_median']} us, flushed "
+            f"{entry['flushed']['kernel_us_median']} us, bit_exact={exact}")
+        del d_src, d_dst
+    del flush
+    res["bit_exact"] = all(res[c[0]]["bit_exact"] for c in prep["cases"])
+    return res
 
 
 # ------------------------------------------------------------------ main
@@ -295,18 +478,47 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
         gc.enable()
         lat = np.zeros(nreq)
         qms = np.zeros(nreq)
-        exact = True
         batch_docs = []
+        held = {}
         for i, t in enumerate(tickets):
             view = t.wait()
-            if i in check:
-                o = offs[i]
-                exact &= bytes(view) == ro.apply_batch(text[o:o + int(sizes[i])], pairs[i])
             lat[i] = lag[i] * 1e3 + t.times["latency_ms"]
             qms[i] = t.times["queue_ms"]
             batch_docs.append(t.times["batch_docs"])
-            t.release()
+            if i in check:
+                held[i] = (t, view)       # checked after the timed section
+            else:
+                t.release()
         wall = time.perf_counter() - t0
+        exact = True
+        for i, (t, view) in held.items():
+            o = offs[i]
+            exact &= bytes(view) == ro.apply_batch(text[o:o + int(sizes[i])], pairs[i])
+            t.release()
+        held.clear()
+        # capacity: the first requests of the same mix submitted at once (closed loop)
+        sat = min(nreq, 2000)
+        t1 = time.perf_counter()
+        burst = [srv.submit_at(text, offs[i], int(sizes[i]), pairs[i]) for i in range(sat)]
+        for t in burst:
+            t.wait()
+        sat_wall = time.perf_counter() - t1
+        for t in burst:
+            t.release()
+        sat_bytes = int(sizes[:sat].sum())
+    # the per-call path on the same request mix: one cuda_replace_unified per pair,
+    # requests in arrival order until about 8 s are spent
+    pc_bytes, pc_reqs = 0, 0
+    t2 = time.perf_counter()
+    for i in range(nreq):
+        d = text[offs[i]:offs[i] + int(sizes[i])]
+        for pat, rep in pairs[i]:
+            d = lib.unified(d, pat, rep)
+        pc_bytes += int(sizes[i])
+        pc_reqs += 1
+        if time.perf_counter() - t2 > 8.0:
+            break
+    pc_wall = time.perf_counter() - t2
     log(f"[bench] serving: {nreq} requests in {wall:.2f}s, p50 {np.percentile(lat, 50):.2f} ms, "
         f"p99 {np.percentile(lat, 99):.2f} ms, bit_exact(sample)={exact}")
     return {
@@ -321,6 +533,16 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
         "max_submit_lag_ms": round(float(lag.max() * 1e3), 3),
         "offered_GBps": round(float(sizes.sum()) / float(arrivals[-1]) / 1e9, 3),
         "achieved_GBps": round(float(sizes.sum()) / wall / 1e9, 3),
+        "achieved_timing": "input bytes of all requests / time from the first scheduled arrival to the last "
+                           "completion (oracle checks run afterwards)",
+        "saturated": {"requests": sat, "input_bytes": sat_bytes, "wall_s": round(sat_wall, 3),
+                      "GBps": round(sat_bytes / sat_wall / 1e9, 3), "req_per_s": round(sat / sat_wall, 1),
+                      "note": "the first requests of the same mix submitted at once"},
+        "per_call": {"requests": pc_reqs, "input_bytes": pc_bytes, "wall_s": round(pc_wall, 3),
+                     "GBps": round(pc_bytes / pc_wall / 1e9, 4), "req_per_s": round(pc_reqs / pc_wall, 2),
+                     "note": "same request mix in arrival order, one cuda_replace_unified per pair, "
+                             "host to host"},
+        "served_vs_per_call": round((sat_bytes / sat_wall) / (pc_bytes / pc_wall), 1),
         "bit_exact": bool(exact),
         "checked_sample": len(check),
     }
@@ -386,6 +608,7 @@ def main() -> int:
     ap.add_argument("--stream-gib", type=int, default=16)
     ap.add_argument("--no-session-batch", action="store_true", help="skip the 128-pair session batch")
     ap.add_argument("--session-gib", type=float, default=2.0)
+    ap.add_argument("--no-c1", action="store_true", help="skip the 64 MiB configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the replace path runs on a single MI355X; "
@@ -419,14 +642,19 @@ def main() -> int:
         expected[name] = out
         cpu_times[name] = best
         log(f"[bench] cpu bytes.replace {name}: {best:.3f}s  ({n / best / 1e9:.2f} GB/s)")
-    procs = min(16, len(os.sched_getaffinity(0)))
+    # worker processes: the host cores this process may run on, capped at the 16
+    # host cores a one-GPU share of the box provides
+    affinity = len(os.sched_getaffinity(0))
+    procs = min(16, affinity)
     par = {}
     if not args.no_cpu:
         for name, key, pat, rep in passes:
-            out, dt, pieces = cpu_parallel(key, pat, rep, procs)
-            assert out == expected[name], name
-            par[name] = (dt, pieces)
-            log(f"[bench] cpu parallel {name}: {dt:.3f}s over {pieces} piece(s)")
+            wall, pieces, slowest, ok = cpu_parallel(key, pat, rep, procs, expected[name])
+            assert ok, name
+            par[name] = (wall, pieces, slowest)
+            log(f"[bench] cpu parallel {name}: {wall:.3f}s over {pieces} piece(s), slowest worker "
+                f"{slowest:.3f}s")
+    c1_prep = None if args.no_c1 else c1_prepare(ro)
     sb_prep = None if args.no_session_batch else session_batch_prepare(ro, args.session_gib, procs)
 
     import torch
@@ -498,6 +726,7 @@ def main() -> int:
         ok = got == expected[name]
         exact &= ok
         log(f"[bench] {name}: out_len={ol} bit_exact={ok}")
+    c1 = None if c1_prep is None else c1_bench(lib, h, c1_prep, dev)
     lib.lib.cuda_replace_close(h)
     batched = None if args.no_batch else batch_bench(lib, ro, args.batch_docs)
     serving = None if args.no_serve else serve_bench(lib, ro, text, args.serve_seconds, workers=args.serve_workers)
@@ -556,13 +785,20 @@ def main() -> int:
                          "sample": f"CPython bytes.replace, single thread, one pass over each of the "
                                    f"three {args.size_mib} MiB inputs (best of 2)",
                          "parallel": ({"value": round(cpu_in / sum(v[0] for v in par.values()) / 1e9, 3),
-                                       "cores": procs,
+                                       "unit": "GB/s", "cores": procs,
+                                       "per_input_GBps": {k: round(n / v[0] / 1e9, 3) for k, v in par.items()},
                                        "pieces": {k: v[1] for k, v in par.items()},
-                                       "note": "fork pool, bytes.replace per safe-boundary chunk; "
-                                               "all-'a' has no safe boundary and runs as one piece"}
+                                       "slowest_worker_s": {k: round(v[2], 3) for k, v in par.items()},
+                                       "note": "one forked process per safe-boundary chunk, started together; "
+                                               "bytes.replace per chunk, results returned as length + "
+                                               "digest and checked against the single-thread output; wall "
+                                               "from start to the last result.  all-'a' has no safe "
+                                               "boundary and runs as one piece"}
                                       if par else None),
-                         "cpu": platform.processor() or platform.machine()},
+                         "cpu": cpu_model(), "affinity_cores": affinity,
+                         "python": platform.python_version()},
         "device_copy_GBps": round(copy_gbs, 1),
+        "c1": c1,
         "batched": batched,
         "serving": serving,
         "streaming": streaming,
@@ -571,7 +807,7 @@ def main() -> int:
         "detail": detail,
     }
     print(json.dumps(line), flush=True)
-    ok = exact and all(x is None or x["bit_exact"] for x in (batched, serving, streaming, session_batch))
+    ok = exact and all(x is None or x["bit_exact"] for x in (c1, batched, serving, streaming, session_batch))
     return 0 if ok else 1
 
 

@@ -2218,29 +2218,33 @@ last_kept_kernel(const TileInfo *info, int64_t num_tiles, unsigned long long *ou
 }
 
 // ---- presence index: 256-bit byte-value mask per chunk -----------------------
+// Grid-stride over the chunks (64-bit indices), so any chunk count fits the grid.
 __global__ void __launch_bounds__(kThreads)
-presence_kernel(const uint8_t *src, int64_t n, int64_t chunk, uint64_t *masks)
+presence_kernel(const uint8_t *src, int64_t n, int64_t chunk, uint64_t *masks, int64_t num_chunks)
 {
     __shared__ unsigned int s_bits[8];
-    if (threadIdx.x < 8)
-        s_bits[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t b = (int64_t)blockIdx.x * chunk;
-    int64_t e = b + chunk;
-    if (e > n)
-        e = n;
-    unsigned int local[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t i = b + threadIdx.x; i < e; i += kThreads) {
-        const uint8_t v = src[i];
-        local[v >> 5] |= 1u << (v & 31);
+    for (int64_t c = blockIdx.x; c < num_chunks; c += gridDim.x) {
+        if (threadIdx.x < 8)
+            s_bits[threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t b = c * chunk;
+        int64_t e = b + chunk;
+        if (e > n)
+            e = n;
+        unsigned int local[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t i = b + threadIdx.x; i < e; i += kThreads) {
+            const uint8_t v = src[i];
+            local[v >> 5] |= 1u << (v & 31);
+        }
+        for (int q = 0; q < 8; ++q)
+            if (local[q])
+                atomicOr(&s_bits[q], local[q]);
+        __syncthreads();
+        if (threadIdx.x < 4)
+            masks[c * 4 + threadIdx.x] =
+                (uint64_t)s_bits[2 * threadIdx.x] | ((uint64_t)s_bits[2 * threadIdx.x + 1] << 32);
+        __syncthreads();   // s_bits is cleared for the next chunk
     }
-    for (int q = 0; q < 8; ++q)
-        if (local[q])
-            atomicOr(&s_bits[q], local[q]);
-    __syncthreads();
-    if (threadIdx.x < 4)
-        masks[blockIdx.x * 4 + threadIdx.x] =
-            (uint64_t)s_bits[2 * threadIdx.x] | ((uint64_t)s_bits[2 * threadIdx.x + 1] << 32);
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -2453,8 +2457,11 @@ hipError_t launch_last_kept(const ReplaceParams &p, int64_t *out1, hipStream_t s
 hipError_t launch_presence_index(const uint8_t *src, int64_t n, int64_t chunk, uint64_t *masks,
                                  int64_t num_chunks, hipStream_t s)
 {
-    hipLaunchKernelGGL(presence_kernel, dim3((unsigned)num_chunks), dim3(kThreads), 0, s, src, n, chunk,
-                       masks);
+    if (num_chunks <= 0)
+        return hipSuccess;
+    const int64_t grid = num_chunks < (1 << 20) ? num_chunks : (1 << 20);
+    hipLaunchKernelGGL(presence_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, src, n, chunk, masks,
+                       num_chunks);
     return hipGetLastError();
 }
 

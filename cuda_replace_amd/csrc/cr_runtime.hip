@@ -1639,6 +1639,11 @@ int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d_dst, s
     if (!s->alive)
         return -3;
     DeviceGuard g(s->device);
+    // the pass reuses the session's pattern buffer and tile records, so the record
+    // of the session's last apply is dropped: a later prefix query answers 0 / -1
+    // until the next apply (the session's buffer itself is untouched)
+    s->q_valid = false;
+    s->fused_pending = false;
     const int64_t f = pat_len > 0 ? floor_from_seed(seed, pat_len) : 0;
     PassResult res;
     auto dst_for = [&](int64_t need, uint8_t **dst) -> int {

@@ -68,16 +68,40 @@ def _u8(b: bytes):
     return C.cast(C.c_char_p(b), c_uint8_p)
 
 
+class _Arr:
+    """A ctypes array with numpy's ``.ctypes.data_as`` spelling, so both forms of
+    _blob_arrays' result are used the same way."""
+
+    def __init__(self, arr):
+        self.arr = arr
+        self.ctypes = self
+
+    def data_as(self, ptr_type):
+        return C.cast(self.arr, ptr_type)
+
+
 def _blob_arrays(items):
     """(blob, pointer array, length array) for a list of bytes: the items joined
-    into one buffer (kept alive by the caller) and the address of each item in it."""
-    import numpy as np
-    lens = np.fromiter(map(len, items), dtype=np.int32, count=len(items))
+    into one buffer (kept alive by the caller) and the address of each item in it.
+    numpy builds the arrays when it is installed; without it (the reference wrapper
+    needs only the standard library, py:1-6) ctypes arrays are built in Python."""
+    k = len(items)
     blob = b"".join(items) + b"\0"
     base = C.cast(C.c_char_p(blob), c_void_p).value
-    offs = np.zeros(max(len(items), 1), dtype=np.uint64)
-    if len(items) > 1:
-        np.cumsum(lens[:-1], dtype=np.uint64, out=offs[1:len(items)])
+    try:
+        import numpy as np
+    except ImportError:
+        np = None
+    if np is None:
+        from itertools import accumulate
+        lens = list(map(len, items))
+        offs = [0] + list(accumulate(lens[:-1])) if k else [0]
+        ptrs = (C.c_uint64 * max(k, 1))(*(base + o for o in offs[:max(k, 1)]))
+        return blob, _Arr(ptrs), _Arr((C.c_int * max(k, 1))(*lens))
+    lens = np.fromiter(map(len, items), dtype=np.int32, count=k)
+    offs = np.zeros(max(k, 1), dtype=np.uint64)
+    if k > 1:
+        np.cumsum(lens[:-1], dtype=np.uint64, out=offs[1:k])
     ptrs = offs + np.uint64(base)
     if len(lens) == 0:
         lens = np.zeros(1, dtype=np.int32)
@@ -86,8 +110,8 @@ def _blob_arrays(items):
 
 def _pair_arrays(pairs):
     """(pats, pat_lens, reps, rep_lens, keepalive) C arrays for a pair list: the
-    patterns and the replacements each joined into one buffer, with numpy arrays of
-    their addresses and lengths (a ctypes object per pair costs ~3 us each)."""
+    patterns and the replacements each joined into one buffer, with arrays of their
+    addresses and lengths (a ctypes object per pair costs ~3 us each)."""
     pblob, pptrs, plens = _blob_arrays([bytes(p) for p, _ in pairs])
     rblob, rptrs, rlens = _blob_arrays([bytes(r) for _, r in pairs])
     return (pptrs.ctypes.data_as(c_uint8_pp), plens.ctypes.data_as(c_int_p), rptrs.ctypes.data_as(c_uint8_pp),
@@ -489,6 +513,15 @@ class Ticket:
             self._server.lib.lib.cr_server_release(self._h)
             self._h = c_void_p()
             self._keep = ()
+
+    def __del__(self):
+        # the queued request reads the bytes this ticket keeps alive: a ticket
+        # dropped unread is released only once its request is done
+        # (cr_server_release waits for it)
+        try:
+            self.release()
+        except Exception:
+            pass
 
     def result(self) -> bytes:
         if self._out is None:

@@ -153,7 +153,10 @@ CR_API int cr_query_prefix64(void *h, size_t limit, long long *kept_before_limit
  * (pass a value <= -pat_len for none).  *out_len gets the output length; if
  * dst_cap is too small nothing is written, *out_len still gets the needed
  * length and hipErrorInvalidValue is returned.  *kernel_ms (optional) gets the
- * device time of the pass measured with HIP events on the session stream. */
+ * device time of the pass measured with HIP events on the session stream.
+ * The pass uses the session's scratch, so the record of the session's last
+ * apply is dropped: a prefix query answers 0 / -1 until the next apply; the
+ * session's own buffer is not touched. */
 CR_API int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d_dst,
                       size_t dst_cap, const uint8_t *pat, int pat_len,
                       const uint8_t *rep, int rep_len, long long seed,
