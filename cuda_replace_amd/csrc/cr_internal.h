@@ -129,6 +129,9 @@ struct ReplaceParams {
     int32_t r;
     uint32_t pat4;         // first min(L,4) pattern bytes, little-endian
     uint32_t mask4;        // byte mask of those bytes
+    uint32_t qref[4];      // masked-SAD reference words: pattern bytes 4q..4q+3 xor qx's byte,
+                           // 0 past the pattern (L <= 16; the SAD skips zero reference bytes)
+    uint32_t qx;           // a byte value absent from the pattern, in all four bytes
     uint32_t *gsum;        // per-group kept count, or LT table entries per group
     ChunkState *chunks;    // num_chunks + 1 chunk states (or spine block states)
     uint64_t *block_tabs;  // spine blocks: each block's summary (kMaxTablePat words)
@@ -282,6 +285,16 @@ hipError_t launch_spine(const ReplaceParams &p, int mode, int table_size, int64_
                         int64_t groups, int64_t chunk, int last_chunk, hipStream_t s);
 hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int gt, int64_t first_group,
                        int64_t groups, hipStream_t s);
+// The lean border-free pass over 16 KiB groups (kGroupTiles tiles): count, then the
+// border-free spine, then emit.
+hipError_t launch_bf_count(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s);
+hipError_t launch_bf_emit(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s);
+// The whole border-free pass in one persistent launch (count ahead, scanner, emit).
+// sweep: sweep_words(num_groups) 64-bit words holding no word tagged with p.epoch;
+// the kept total lands in p.stats (tagged), or sweep[2 * groups] = epoch << 48 | 1
+// when a wait gave up (then nothing of the pass is valid: run the three launches).
+hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, int spin_limit, hipStream_t s);
+int64_t sweep_words(int64_t groups);
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_empty_pattern(const ReplaceParams &p, hipStream_t s);

@@ -28,6 +28,7 @@
 //   otherwise: candidate bitmap, one-wave serial greedy, then the emit kernel scatters.
 #include "cr_internal.h"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -1221,6 +1222,884 @@ emit_kernel(ReplaceParams p, int64_t first_group)
     emit_group<MODE, LT, GT>(p, g, p.chunks_pre + g);
 }
 
+// ---- lean border-free pass: 64 source bytes per thread ------------------------
+// The pass of a border-free pattern (no proper prefix equals a suffix: occurrences
+// never overlap, so every occurrence at or above the seed floor is kept).  A
+// workgroup takes a 16 KiB group; thread t owns source bytes [64t, 64t + 64) of it.
+//   bf_count  stages the group, builds each thread's 64-position candidate mask and
+//             writes the group's kept count (one word);
+//   spine     the same spine as the other modes (group states);
+//   bf_emit   stages the group again, rebuilds the masks, scans kept counts and last
+//             kept starts across the workgroup (wave64 DPP scans), and writes every
+//             thread's pieces (source runs and replacements) into an LDS image of
+//             the group's output, which the workgroup then stores with 16-byte
+//             stores.
+// A piece copy moves 16-byte chunks with its last chunk ending exactly at the run
+// end (the chunks overlap; same bytes), so only runs shorter than 16 bytes take the
+// 8/4/2/1-byte forms.
+constexpr int kBfSeg = 64;                            // source bytes per thread
+constexpr int kBfGroup = kThreads * kBfSeg;           // 16384 source bytes per workgroup
+static_assert(kBfGroup == kGroupBytes, "the spine's group size");
+constexpr int kBfOutCap = kBfGroup + 4096;            // LDS image of a group's output
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// wave64 inclusive scans with DPP (row_shr 1/2/4/8 inside rows of 16 lanes, then
+// row_bcast 15 and 31 across rows).  Lanes without a source read the identity.
+__device__ __forceinline__ uint32_t wave_incl_add_u32(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+
+__device__ __forceinline__ int wave_incl_max_i32(int x)
+{
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x118, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x142, 0xa, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+
+// previous lane's value (wave_shr 1); lane 0 reads `first`
+__device__ __forceinline__ int wave_shr1_i32(int x, int first)
+{
+    return __builtin_amdgcn_update_dpp(first, x, 0x138, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ uint64_t lds_load8(const uint8_t *p)
+{
+    uint64_t v;
+    __builtin_memcpy(&v, p, 8);
+    return v;
+}
+__device__ __forceinline__ void lds_store8(uint8_t *p, uint64_t v) { __builtin_memcpy(p, &v, 8); }
+__device__ __forceinline__ uint32_t lds_load4(const uint8_t *p)
+{
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+__device__ __forceinline__ void lds_store4(uint8_t *p, uint32_t v) { __builtin_memcpy(p, &v, 4); }
+__device__ __forceinline__ uint16_t lds_load2(const uint8_t *p)
+{
+    uint16_t v;
+    __builtin_memcpy(&v, p, 2);
+    return v;
+}
+__device__ __forceinline__ void lds_store2(uint8_t *p, uint16_t v) { __builtin_memcpy(p, &v, 2); }
+
+// Copies len (0..64) bytes between LDS buffers at any alignment, exactly: 16-byte
+// chunks with the last one ending at len, or two overlapping 8/4/2-byte moves.
+__device__ __forceinline__ void bf_copy_run(uint8_t *dst, const uint8_t *src, int len)
+{
+    if (len >= 16) {
+        for (int k = 0; k < len - 16; k += 16)
+            lds_store16(dst + k, lds_load16(src + k));
+        lds_store16(dst + len - 16, lds_load16(src + len - 16));
+    } else if (len >= 8) {
+        const uint64_t a = lds_load8(src), b = lds_load8(src + len - 8);
+        lds_store8(dst, a);
+        lds_store8(dst + len - 8, b);
+    } else if (len >= 4) {
+        const uint32_t a = lds_load4(src), b = lds_load4(src + len - 4);
+        lds_store4(dst, a);
+        lds_store4(dst + len - 4, b);
+    } else if (len >= 2) {
+        const uint16_t a = lds_load2(src), b = lds_load2(src + len - 2);
+        lds_store2(dst, a);
+        lds_store2(dst + len - 2, b);
+    } else if (len == 1) {
+        *dst = *src;
+    }
+}
+
+// The replacement split once into the 8/4/2/1-byte stores of its length (r < 16);
+// longer replacements are copied from LDS (r <= kRepLds) or HBM.
+struct BfRep {
+    uint64_t w8;
+    uint32_t w4;
+    uint32_t w2;
+    uint32_t w1;
+    int o4, o2, o1;
+};
+
+__device__ __forceinline__ BfRep bf_rep_parts(uint4 v, int r)
+{
+    BfRep b;
+    uint64_t lo = (uint64_t)v.x | ((uint64_t)v.y << 32);
+    const uint64_t hi = (uint64_t)v.z | ((uint64_t)v.w << 32);
+    b.w8 = lo;
+    int off = 0;
+    if (r & 8) {
+        lo = hi;
+        off = 8;
+    }
+    b.o4 = off;
+    b.w4 = (uint32_t)lo;
+    if (r & 4) {
+        lo >>= 32;
+        off += 4;
+    }
+    b.o2 = off;
+    b.w2 = (uint32_t)lo & 0xffffu;
+    if (r & 2) {
+        lo >>= 16;
+        off += 2;
+    }
+    b.o1 = off;
+    b.w1 = (uint32_t)lo & 0xffu;
+    return b;
+}
+
+__device__ __forceinline__ void bf_put_rep_lds(uint8_t *dst, const BfRep &b, uint4 v, const uint8_t *s_rep, int r)
+{
+    if (r < 16) {
+        if (r & 8)
+            lds_store8(dst, b.w8);
+        if (r & 4)
+            lds_store4(dst + b.o4, b.w4);
+        if (r & 2)
+            lds_store2(dst + b.o2, (uint16_t)b.w2);
+        if (r & 1)
+            dst[b.o1] = (uint8_t)b.w1;
+    } else if (r == 16) {
+        lds_store16(dst, v);
+    } else {
+        bf_copy_run(dst, s_rep, r <= 64 ? r : 64);
+        for (int k = 64; k < r; k += 64)
+            bf_copy_run(dst + k, s_rep + k, r - k < 64 ? r - k : 64);
+    }
+}
+
+// Copies len (0..64) bytes from LDS to HBM (any alignment), exactly: the 16-byte
+// chunks are read first (the last one ending at len), then stored; a run shorter than
+// 16 bytes is stored from one read as 8/4/2/1-byte pieces.
+__device__ __forceinline__ void bf_store_run(uint8_t *dst, const uint8_t *src, int len)
+{
+    if (len >= 16) {
+        const int e = len - 16;
+        const uint4 a = lds_load16(src), b = lds_load16(src + (16 < e ? 16 : e)),
+                    c = lds_load16(src + (32 < e ? 32 : e)), d = lds_load16(src + e);
+        __builtin_memcpy(dst, &a, 16);
+        __builtin_memcpy(dst + (16 < e ? 16 : e), &b, 16);
+        __builtin_memcpy(dst + (32 < e ? 32 : e), &c, 16);
+        __builtin_memcpy(dst + e, &d, 16);
+    } else if (len > 0) {
+        store_tail(dst, lds_load16(src), len);
+    }
+}
+
+__device__ __forceinline__ void bf_put_rep_global(uint8_t *dst, const BfRep &b, uint4 v, const uint8_t *s_rep,
+                                                  const uint8_t *g_rep, int r)
+{
+    if (r < 16) {
+        if (r & 8)
+            __builtin_memcpy(dst, &b.w8, 8);
+        if (r & 4)
+            __builtin_memcpy(dst + b.o4, &b.w4, 4);
+        if (r & 2) {
+            const uint16_t h = (uint16_t)b.w2;
+            __builtin_memcpy(dst + b.o2, &h, 2);
+        }
+        if (r & 1)
+            dst[b.o1] = (uint8_t)b.w1;
+    } else if (r == 16) {
+        __builtin_memcpy(dst, &v, 16);
+    } else if (r <= kRepLds) {
+        for (int k = 0; k < r; k += 64)
+            bf_store_run(dst + k, s_rep + k, r - k < 64 ? r - k : 64);
+    } else {
+        store_from_global(dst, g_rep, r);
+    }
+}
+
+// Candidate mask of the 64 positions whose bytes start at s_seg (16-byte aligned
+// LDS): bit j set iff the pattern occurs at gpos + j and floor <= gpos + j <= n - L.
+// FULL4: L >= 4, so the first four pattern bytes are compared unmasked.  Reads
+// s_seg[0, 68) and, for L > 4, the candidates' bytes up to L - 1 past them.
+template <bool FULL4>
+__device__ __forceinline__ uint64_t bf_cand64(const ReplaceParams &p, const uint8_t *s_seg, const uint8_t *s_pat,
+                                              int64_t gpos)
+{
+    uint32_t d[17];
+    const uint4 *q = reinterpret_cast<const uint4 *>(s_seg);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 v = q[k];
+        d[4 * k] = v.x;
+        d[4 * k + 1] = v.y;
+        d[4 * k + 2] = v.z;
+        d[4 * k + 3] = v.w;
+    }
+    d[16] = *reinterpret_cast<const uint32_t *>(s_seg + 64);
+    const uint32_t pat4 = p.pat4, mask4 = p.mask4;
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int j = 31; j >= 0; --j) {
+        const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
+        lo = lo + lo + (uint32_t)(FULL4 ? w == pat4 : ((w ^ pat4) & mask4) == 0u);
+    }
+#pragma unroll
+    for (int j = 63; j >= 32; --j) {
+        const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
+        hi = hi + hi + (uint32_t)(FULL4 ? w == pat4 : ((w ^ pat4) & mask4) == 0u);
+    }
+    uint64_t m = ((uint64_t)hi << 32) | lo;
+    const int64_t lo_lim = p.floor - gpos;
+    const int64_t hi_lim = p.n - (int64_t)p.L - gpos;
+    if (lo_lim > 0 || hi_lim < 63) {
+        if (hi_lim < 0 || lo_lim > 63 || lo_lim > hi_lim) {
+            m = 0;
+        } else {
+            const int a = lo_lim < 0 ? 0 : (int)lo_lim;
+            const int b = hi_lim > 63 ? 63 : (int)hi_lim;
+            m &= (b == 63 ? ~0ull : ((2ull << b) - 1ull)) & (~0ull << a);
+        }
+    }
+    if (p.L > 4 && m) {
+        uint64_t mm = m;
+        m = 0;
+        while (mm) {
+            const int j = (int)__builtin_ctzll(mm);
+            mm &= mm - 1;
+            bool ok = true;
+            for (int k = 4; k < p.L; ++k) {
+                if (s_seg[j + k] != s_pat[k]) {
+                    ok = false;
+                    break;
+                }
+            }
+            m |= (uint64_t)ok << j;
+        }
+    }
+    return m;
+}
+
+// Number of occurrences among the 64 positions whose bytes start at s_seg (16-byte
+// aligned LDS), for a pattern of 4(NQ-1) < L <= 4 NQ bytes, all positions in range.
+// v_mqsad_pk_u16_u8 gives the masked byte SADs of the four windows starting at
+// bytes 0..3 of a 64-bit operand against a reference word; chained over the NQ
+// pattern words (the data xor qx, see pattern_qsad) a window's sum is 0 exactly when
+// the pattern occurs there, and (0x8000 - sum) has bit 15 set only then (sums stay
+// below 16 * 255).  Reads s_seg[0, 64 + 4 NQ).
+template <int NQ>
+__device__ __forceinline__ uint32_t bf_count64_qsad(const ReplaceParams &p, const uint8_t *s_seg)
+{
+    uint32_t d[20];
+    const u32x4 *q4 = reinterpret_cast<const u32x4 *>(s_seg);
+    const uint32_t x = p.qx;
+#pragma unroll
+    for (int k = 0; k < (16 + NQ + 3) / 4; ++k) {
+        const u32x4 v = q4[k];
+        d[4 * k] = v.x ^ x;
+        d[4 * k + 1] = v.y ^ x;
+        d[4 * k + 2] = v.z ^ x;
+        d[4 * k + 3] = v.w ^ x;
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            acc = __builtin_amdgcn_mqsad_pk_u16_u8(((uint64_t)d[k + q + 1] << 32) | d[k + q], p.qref[q], acc);
+        const uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
+        cnt += __builtin_popcount((0x80008000u - lo) & 0x80008000u) +
+               __builtin_popcount((0x80008000u - hi) & 0x80008000u);
+    }
+    return cnt;
+}
+
+template <int NQ>
+__device__ __forceinline__ uint32_t bf_count_seg(const ReplaceParams &p, const uint8_t *s_seg, const uint8_t *s_pat,
+                                                 int64_t gpos)
+{
+    // a segment wholly inside [floor, n - L] takes the SAD count; the boundary
+    // segments the per-position masks
+    if (gpos >= p.floor && gpos + 63 <= p.n - p.L)
+        return bf_count64_qsad<NQ>(p, s_seg);
+    return (uint32_t)__builtin_popcountll(bf_cand64<(NQ > 1)>(p, s_seg, s_pat, gpos));
+}
+
+template <bool FULL4>
+__global__ void __launch_bounds__(kThreads)
+bf_count_kernel(ReplaceParams p, int64_t first_group)
+{
+    __shared__ __align__(16) uint8_t s_in[GroupShape<kGroupTiles>::kIn];
+    __shared__ uint8_t s_pat[kMaxSinglePassPat];
+    __shared__ uint32_t s_red[kWaves];
+    const int64_t g = first_group + blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t t0 = g * kGroupTiles;
+    if (t0 >= p.num_tiles)
+        return;
+    if (p.L > 4)
+        for (int i = tid; i < p.L; i += kThreads)
+            s_pat[i] = p.pat[i];
+    load_group<kGroupTiles>(p, t0, tid, false, s_in);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int64_t gpos = t0 * kTile + (int64_t)tid * kBfSeg;
+    uint32_t cnt = 0;
+    if (gpos < p.n) {
+        const uint8_t *seg = s_in + kHalo + tid * kBfSeg;
+        if (p.L <= 4)
+            cnt = bf_count_seg<1>(p, seg, s_pat, gpos);
+        else if (p.L <= 8)
+            cnt = bf_count_seg<2>(p, seg, s_pat, gpos);
+        else if (p.L <= 12)
+            cnt = bf_count_seg<3>(p, seg, s_pat, gpos);
+        else if (p.L <= 16)
+            cnt = bf_count_seg<4>(p, seg, s_pat, gpos);
+        else
+            cnt = (uint32_t)__builtin_popcountll(bf_cand64<FULL4>(p, seg, s_pat, gpos));
+    }
+    cnt = wave_incl_add_u32(cnt);
+    if ((tid & (kWave - 1)) == kWave - 1)
+        s_red[tid / kWave] = cnt;
+    __syncthreads();
+    if (tid == 0)
+        p.gsum[g] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+// One group's source, prefetched into registers: lane l of wave w holds the 16-byte
+// granules w*4096 + 1024k + 16l (k < 4) of the group (coalesced loads), wave 0 the
+// 256 bytes left of the group and wave 1 the 256 bytes right of it (4 per lane).
+// (clang vector types and plain scalars: with HIP's uint4 and the ChunkState struct
+// the stage was kept in scratch memory across the loop iteration)
+struct BfStage {
+    u32x4 v0, v1, v2, v3;
+    uint32_t halo;
+    int64_t k_base;         // the group's ChunkState.k_base (WITH_STATE)
+};
+
+template <bool WITH_STATE>
+__device__ __forceinline__ void bf_prefetch(const ReplaceParams &p, int64_t g, int tid, BfStage &b)
+{
+    // every load is unconditional (a granule past n reads the group's first granule
+    // instead), so no branch makes the compiler wait for the loads where they join
+    const int64_t G0 = g * (int64_t)kBfGroup;
+    const int64_t avail = p.n - G0;
+    const int wave = tid / kWave, lane = tid & (kWave - 1);
+    const uint8_t *const base = p.src + G0;
+    const int off = wave * 4096 + lane * 16;
+    b.v0 = *reinterpret_cast<const u32x4 *>(base + (off < avail ? off : 0));
+    b.v1 = *reinterpret_cast<const u32x4 *>(base + (off + 1024 < avail ? off + 1024 : 0));
+    b.v2 = *reinterpret_cast<const u32x4 *>(base + (off + 2048 < avail ? off + 2048 : 0));
+    b.v3 = *reinterpret_cast<const u32x4 *>(base + (off + 3072 < avail ? off + 3072 : 0));
+    int64_t hoff = lane * 4;                       // wave 0: left halo, wave 1: right halo
+    if (wave == 0)
+        hoff = G0 >= kHalo ? hoff - kHalo : hoff;
+    else if (kBfGroup + lane * 4 < avail)
+        hoff = kBfGroup + lane * 4;
+    b.halo = *reinterpret_cast<const uint32_t *>(base + hoff);
+    if constexpr (WITH_STATE)
+        b.k_base = p.chunks_pre[g].k_base;
+}
+
+__device__ __forceinline__ void bf_stage_to_lds(const BfStage &b, int tid, uint8_t *s_in)
+{
+    const int wave = tid / kWave, lane = tid & (kWave - 1);
+    uint8_t *const d = s_in + kHalo + wave * 4096 + lane * 16;
+    *reinterpret_cast<u32x4 *>(d) = b.v0;
+    *reinterpret_cast<u32x4 *>(d + 1024) = b.v1;
+    *reinterpret_cast<u32x4 *>(d + 2048) = b.v2;
+    *reinterpret_cast<u32x4 *>(d + 3072) = b.v3;
+    if (wave == 0)
+        *reinterpret_cast<uint32_t *>(s_in + lane * 4) = b.halo;
+    if (wave == 1)
+        *reinterpret_cast<uint32_t *>(s_in + kHalo + kBfGroup + lane * 4) = b.halo;
+}
+
+// LDS of a border-free emit workgroup.
+struct BfLds {
+    uint8_t *in;         // the group with halos (GroupShape<kGroupTiles>::kIn)
+    uint8_t *out;        // the group's output image (kBfOutCap)
+    const uint8_t *pat;
+    const uint8_t *rep;
+    uint32_t *wsum;      // kWaves
+    int *wmax;           // kWaves
+    int *halo_last;      // set to INT_MIN before the barrier that publishes the staged group
+    int *tile;           // 4
+};
+
+// Emit of the staged group g, whose kept starts before it number k_excl: per-tile
+// records and the output.  Ends after the output stores are issued; the caller
+// synchronizes before the LDS is reused.
+template <bool FULL4, bool IMAGE>
+__device__ __forceinline__ void bf_emit_staged(const ReplaceParams &p, int64_t g, int64_t k_excl, const BfLds &s,
+                                               uint4 rep_v, const BfRep &rp)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int L = p.L;
+    const int r = p.r;
+    const int64_t n = p.n;
+    const int64_t diff = (int64_t)r - L;
+    const int64_t t0 = g * kGroupTiles;
+    const int64_t G0 = t0 * kTile;
+    const int valid = (int)((n - G0) < kBfGroup ? (n - G0) : kBfGroup);
+    const bool last_group = G0 + kBfGroup >= n;
+
+    // carry into the group: an occurrence in the L-1 bytes left of it (kept unless
+    // below the floor, since occurrences never overlap)
+    if (G0 > 0 && tid < L - 1) {
+        const int qq = -1 - tid;
+        const int64_t gp = G0 + qq;
+        if (gp >= p.floor && gp + L <= n) {
+            const uint8_t *sp = s.in + kHalo + qq;
+            bool ok = true;
+            for (int k = 0; k < L; ++k) {
+                if (sp[k] != s.pat[k]) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok)
+                atomicMax(s.halo_last, qq);
+        }
+    }
+    const int seg = tid * kBfSeg;
+    const int64_t gpos = G0 + seg;
+    const uint64_t m = gpos < n ? bf_cand64<FULL4>(p, s.in + kHalo + seg, s.pat, gpos) : 0ull;
+    const int k_t = __builtin_popcountll(m);
+    const int lq = m ? seg + 63 - (int)__builtin_clzll(m) : INT_MIN;   // last kept start (group-relative)
+    const uint32_t kinc = wave_incl_add_u32((uint32_t)k_t);
+    const int linc = wave_incl_max_i32(lq);
+    if (lane == kWave - 1) {
+        s.wsum[wave] = kinc;
+        s.wmax[wave] = linc;
+    }
+    __syncthreads();
+    const int c0 = *s.halo_last == INT_MIN ? 0 : *s.halo_last + L;   // leading group bytes covered
+    uint32_t kbefore = kinc - (uint32_t)k_t;
+    int blk_prev = wave_shr1_i32(linc, INT_MIN);   // last kept start before my segment
+    uint32_t ktot = 0;
+    int gl = INT_MIN;   // last kept start of the group
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint32_t ws = s.wsum[w];
+        const int wm = s.wmax[w];
+        if (w < wave) {
+            kbefore += ws;
+            blk_prev = max(blk_prev, wm);
+        }
+        ktot += ws;
+        gl = max(gl, wm);
+    }
+    const int prev_last = max(blk_prev, c0 - L);   // with the virtual kept start covering [0, c0)
+    int c_t = prev_last + L - seg;                 // leading bytes of my segment covered
+    c_t = c_t < 0 ? 0 : c_t;
+    const int c_next = (last_group || gl == INT_MIN) ? 0 : max(0, gl + L - kBfGroup);
+
+    // per-tile records: tile 0 is threads 0-127, tile 1 threads 128-255
+    if (tid == 128) {
+        s.tile[0] = (int)kbefore;
+        s.tile[1] = c_t;
+    }
+    if (tid == 127 || tid == 255)
+        s.tile[2 + (tid >> 7)] = max(blk_prev, lq);   // last kept start up to the tile's end
+    // output of the group: [out_base, out_base + out_len) in the destination
+    const int64_t out_len = (int64_t)valid - c0 + c_next + (int64_t)ktot * diff;
+    const int64_t out_base = G0 + c0 + k_excl * diff;
+    const int jend = (valid - seg) < kBfSeg ? ((valid - seg) > 0 ? valid - seg : 0) : kBfSeg;
+    int j = c_t < jend ? c_t : jend;
+    const int64_t o = (int64_t)(seg + j - c0) + (int64_t)kbefore * diff;   // my first output byte
+    uint64_t M = m;
+    __syncthreads();   // s.tile
+
+    if (tid == 0 && p.info) {
+        TileInfo a;
+        a.k_excl = k_excl;
+        a.last_abs = s.tile[2] >= 0 ? G0 + s.tile[2] : -1;
+        a.carry_in = c0;
+        a.mode = kModeBorderFree;
+        p.info[t0] = a;
+        if (t0 + 1 < p.num_tiles) {
+            TileInfo b;
+            b.k_excl = k_excl + s.tile[0];
+            b.last_abs = s.tile[3] >= kTile ? G0 + s.tile[3] : -1;
+            b.carry_in = s.tile[1] < kTile ? s.tile[1] : kTile;
+            b.mode = kModeBorderFree;
+            p.info[t0 + 1] = b;
+        }
+    }
+    if (!p.write_output)
+        return;
+
+    const uint8_t *const s_src = s.in + kHalo + seg;
+    if (!IMAGE) {
+        // my pieces straight to HBM: runs as 16-byte stores with the last one ending at
+        // the run's end (or one exact tail), replacements from registers
+        uint8_t *out = p.dst + out_base + o;
+        for (;;) {
+            const int qk = M ? (int)__builtin_ctzll(M) : jend;
+            bf_store_run(out, s_src + j, qk - j);
+            out += qk - j;
+            if (qk >= jend)
+                break;
+            bf_put_rep_global(out, rp, rep_v, s.rep, p.rep, r);
+            out += r;
+            j = qk + L;
+            M &= M - 1;
+            if (j >= jend)
+                break;
+        }
+    } else if (out_len + 16 <= kBfOutCap && r <= kRepLds) {
+        // my pieces into the LDS image, then the workgroup stores the image
+        int oo = (int)o;
+        for (;;) {
+            const int qk = M ? (int)__builtin_ctzll(M) : jend;
+            bf_copy_run(s.out + oo, s_src + j, qk - j);
+            oo += qk - j;
+            if (qk >= jend)
+                break;
+            bf_put_rep_lds(s.out + oo, rp, rep_v, s.rep, r);
+            oo += r;
+            j = qk + L;
+            M &= M - 1;
+            if (j >= jend)
+                break;
+        }
+        __syncthreads();
+        uint8_t *const out_first = p.dst + out_base;
+        const int len = (int)out_len;
+        for (int c = tid * 16; c < len; c += kThreads * 16) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(s.out + c);
+            if (c + 16 <= len)
+                __builtin_memcpy(out_first + c, &v, 16);
+            else
+                store_tail(out_first + c, v, len - c);
+        }
+    } else {
+        // a group whose output does not fit the image (or a long replacement): every
+        // thread writes its pieces straight to HBM
+        uint8_t *out = p.dst + out_base + o;
+        for (;;) {
+            const int qk = M ? (int)__builtin_ctzll(M) : jend;
+            if (qk > j)
+                store_from_lds(out, s_src, j, qk - j);
+            out += qk - j;
+            if (qk >= jend)
+                break;
+            if (r <= kRepLds)
+                store_from_lds(out, s.rep, 0, r);
+            else
+                store_from_global(out, p.rep, r);
+            out += r;
+            j = qk + L;
+            M &= M - 1;
+            if (j >= jend)
+                break;
+        }
+    }
+}
+
+// The LDS arrays of a border-free emit workgroup, declared in the kernel.
+#define BF_EMIT_LDS                                                      \
+    __shared__ __align__(16) uint8_t s_in[GroupShape<kGroupTiles>::kIn]; \
+    __shared__ __align__(16) uint8_t s_out[kBfOutCap];                   \
+    __shared__ uint8_t s_pat[kMaxSinglePassPat];                         \
+    __shared__ __align__(16) uint8_t s_rep[kRepLds + 64];                \
+    __shared__ uint32_t s_wsum[kWaves];                                  \
+    __shared__ int s_wmax[kWaves];                                       \
+    __shared__ int s_halo_last;                                          \
+    __shared__ int s_tile[4];                                            \
+    const BfLds lds{s_in, s_out, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
+
+// Persistent emit: workgroup w takes groups first_group + w, + gridDim.x, ... and
+// prefetches each next group's source into registers while it works on the current
+// one, so the loads are in flight during the computation instead of before it.
+// Workgroups never wait on each other.
+template <bool FULL4>
+__global__ void __launch_bounds__(kThreads)
+bf_emit_kernel(ReplaceParams p, int64_t first_group, int64_t groups)
+{
+    BF_EMIT_LDS
+    const int tid = threadIdx.x;
+    const int64_t g_end = first_group + groups;
+    for (int i = tid; i < p.L; i += kThreads)
+        s_pat[i] = p.pat[i];
+    for (int i = tid; i < p.r && i < kRepLds; i += kThreads)
+        s_rep[i] = p.rep[i];
+    int64_t g = first_group + blockIdx.x;
+    BfStage stage;
+    bf_prefetch<true>(p, g < g_end ? g : g_end - 1, tid, stage);
+    __syncthreads();
+    const uint4 rep_v = lds_load16(s_rep);
+    const BfRep rp = bf_rep_parts(rep_v, p.r);
+
+    for (; g < g_end; g += gridDim.x) {
+        // the group's kept-before count came with the prefetch; the empty asm keeps it in
+        // vector registers until here (held in scalar registers, it was waited for
+        // right after the prefetch was issued)
+        int64_t k_excl = stage.k_base;
+        asm volatile("" : "+v"(k_excl));
+        __syncthreads();                       // the previous group is done with s_in / s_out
+        bf_stage_to_lds(stage, tid, s_in);
+        if (tid == 0)
+            s_halo_last = INT_MIN;
+        // the next group's source (the last iteration loads its own group again)
+        bf_prefetch<true>(p, g + gridDim.x < g_end ? g + gridDim.x : g, tid, stage);
+        __syncthreads();
+        bf_emit_staged<FULL4, true>(p, g, k_excl, lds, rep_v, rp);
+    }
+}
+
+// The LDS of a border-free workgroup without an output image.
+#define BF_DIRECT_LDS                                                    \
+    __shared__ __align__(16) uint8_t s_in[GroupShape<kGroupTiles>::kIn]; \
+    __shared__ uint8_t s_pat[kMaxSinglePassPat];                         \
+    __shared__ __align__(16) uint8_t s_rep[kRepLds + 64];                \
+    __shared__ uint32_t s_wsum[kWaves];                                  \
+    __shared__ int s_wmax[kWaves];                                       \
+    __shared__ int s_halo_last;                                          \
+    __shared__ int s_tile[4];                                            \
+    const BfLds lds{s_in, nullptr, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
+
+// Emit with one group per workgroup and no output image: the group comes in by LDS
+// DMA and every thread stores its pieces itself.  The small LDS footprint lets twice
+// as many workgroups share a compute unit as with an image.
+template <bool FULL4>
+__global__ void __launch_bounds__(kThreads)
+bf_emit_direct_kernel(ReplaceParams p, int64_t first_group)
+{
+    BF_DIRECT_LDS
+    const int tid = threadIdx.x;
+    const int64_t g = first_group + blockIdx.x;
+    const int64_t t0 = g * kGroupTiles;
+    if (t0 >= p.num_tiles)
+        return;
+    for (int i = tid; i < p.L; i += kThreads)
+        s_pat[i] = p.pat[i];
+    for (int i = tid; i < p.r && i < kRepLds; i += kThreads)
+        s_rep[i] = p.rep[i];
+    if (tid == 0)
+        s_halo_last = INT_MIN;
+    load_group<kGroupTiles>(p, t0, tid, true, s_in);
+    const int64_t k_excl = p.chunks_pre[g].k_base;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const uint4 rep_v = lds_load16(s_rep);
+    const BfRep rp = bf_rep_parts(rep_v, p.r);
+    bf_emit_staged<FULL4, false>(p, g, k_excl, lds, rep_v, rp);
+}
+
+// ---- single-pass border-free sweep -------------------------------------------------
+// One persistent launch does the whole pass.  Workgroup 0 is the scanner: its first
+// wave turns the groups' kept counts into exclusive prefixes, 64 groups per step with
+// 16 steps of loads in flight.  Every other workgroup is a worker w < W = grid - 1 and
+// owns group kW + w of round k.  Its phases alternate: the count phase of round k
+// (stage, count, publish the count) runs `ahead` rounds before the emit phase of the
+// same group (stage it again -- it was read `ahead` rounds earlier, so the read
+// normally hits the memory-side cache -- wait for its prefix, emit).  A phase's data
+// is prefetched into registers while the previous phase computes.  Counts and
+// prefixes are 64-bit words tagged with the pass's epoch (relaxed agent-scope
+// atomics).  Every wait is bounded: one that runs out sets the abort word, every
+// waiter sees it and returns, and the host redoes the pass with the three launches.
+struct SweepArgs {
+    uint64_t *agg;        // per group: epoch << 48 | kept count
+    uint64_t *pref;       // per group: epoch << 48 | kept starts before the group
+    uint64_t *flag;       // epoch << 48 | 1 once a wait has given up
+    int32_t ahead;        // rounds the count phase runs ahead of the emit phase
+    int32_t spin_limit;   // polls before a wait gives up
+};
+
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr uint64_t kTagMask = ~((1ull << kEpochShift) - 1);
+
+__device__ __forceinline__ void bf_scanner(const ReplaceParams &p, const SweepArgs &a, int lane, uint64_t tag,
+                                           int64_t G)
+{
+    constexpr int KB = 16;
+    uint64_t running = 0;
+    const int64_t nb = (G + kWave - 1) / kWave;
+    for (int64_t b0 = 0; b0 < nb; b0 += KB) {
+        uint64_t v[KB];
+#pragma unroll
+        for (int i = 0; i < KB; ++i) {
+            const int64_t idx = (b0 + i) * kWave + lane;
+            v[i] = ld_agent(a.agg + (idx < G ? idx : G - 1));
+        }
+#pragma unroll
+        for (int i = 0; i < KB; ++i) {
+            if (b0 + i >= nb)
+                break;
+            const int64_t idx = (b0 + i) * kWave + lane;
+            uint64_t x = v[i];
+            int spins = 0;
+            for (;;) {
+                const bool ok = idx >= G || (x & kTagMask) == tag;
+                if (__builtin_amdgcn_ballot_w64(!ok) == 0)
+                    break;
+                if (++spins > a.spin_limit || (spins % 64 == 0 && ld_agent(a.flag) == (tag | 1))) {
+                    if (lane == 0)
+                        st_agent(a.flag, tag | 1);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                if (!ok)
+                    x = ld_agent(a.agg + idx);
+            }
+            const uint32_t c = idx < G ? (uint32_t)(x & 0xffffffffu) : 0u;
+            const uint32_t inc = wave_incl_add_u32(c);
+            if (idx < G)
+                st_agent(a.pref + idx, tag | (running + inc - c));
+            running += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+        }
+    }
+    if (lane == 0)
+        st_agent(&p.stats->k_total, tag | running);
+}
+
+// Kept count of the staged group (all four waves), published by thread 0.
+__device__ __forceinline__ void bf_count_staged(const ReplaceParams &p, int64_t g, const BfLds &s,
+                                                const SweepArgs &a, uint64_t tag)
+{
+    const int tid = threadIdx.x;
+    const int64_t gpos = g * (int64_t)kBfGroup + (int64_t)tid * kBfSeg;
+    uint32_t cnt = 0;
+    if (gpos < p.n) {
+        const uint8_t *seg = s.in + kHalo + tid * kBfSeg;
+        if (p.L <= 4)
+            cnt = bf_count_seg<1>(p, seg, s.pat, gpos);
+        else if (p.L <= 8)
+            cnt = bf_count_seg<2>(p, seg, s.pat, gpos);
+        else if (p.L <= 12)
+            cnt = bf_count_seg<3>(p, seg, s.pat, gpos);
+        else if (p.L <= 16)
+            cnt = bf_count_seg<4>(p, seg, s.pat, gpos);
+        else
+            cnt = (uint32_t)__builtin_popcountll(bf_cand64<true>(p, seg, s.pat, gpos));
+    }
+    cnt = wave_incl_add_u32(cnt);
+    if ((tid & (kWave - 1)) == kWave - 1)
+        s.wsum[tid / kWave] = cnt;
+    __syncthreads();
+    if (tid == 0)
+        st_agent(a.agg + g, tag | (uint64_t)(s.wsum[0] + s.wsum[1] + s.wsum[2] + s.wsum[3]));
+}
+
+template <bool FULL4, bool IMAGE>
+__global__ void __launch_bounds__(kThreads)
+bf_sweep_kernel(ReplaceParams p, SweepArgs a)
+{
+    __shared__ __align__(16) uint8_t s_in[GroupShape<kGroupTiles>::kIn];
+    __shared__ __align__(16) uint8_t s_out_mem[IMAGE ? kBfOutCap : 16];
+    __shared__ uint8_t s_pat[kMaxSinglePassPat];
+    __shared__ __align__(16) uint8_t s_rep[kRepLds + 64];
+    __shared__ uint32_t s_wsum[kWaves];
+    __shared__ int s_wmax[kWaves];
+    __shared__ int s_halo_last;
+    __shared__ int s_tile[4];
+    const BfLds lds{s_in, IMAGE ? s_out_mem : nullptr, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
+    __shared__ int64_t s_kexcl;
+    __shared__ int s_abort;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    for (int i = tid; i < p.L; i += kThreads)
+        s_pat[i] = p.pat[i];
+    for (int i = tid; i < p.r && i < kRepLds; i += kThreads)
+        s_rep[i] = p.rep[i];
+    const uint64_t tag = (uint64_t)p.epoch << kEpochShift;
+    const int64_t G = p.num_groups;
+    if (blockIdx.x == 0) {
+        if (tid < kWave)
+            bf_scanner(p, a, lane, tag, G);
+        return;
+    }
+    const int64_t W = gridDim.x - 1;
+    const int64_t w = blockIdx.x - 1;
+    const int64_t D = a.ahead;
+    const int64_t R = w < G ? (G - w + W - 1) / W : 0;   // rounds in which this worker has a group
+    const int64_t Dp = D < R ? D : R;                   // count phases before the first emit
+    const int64_t mid = R > D ? R - D : 0;              // rounds with a count and an emit phase
+    const int64_t phases = 2 * R;
+    // phase i: count phases of rounds 0..Dp-1, then (count k, emit k-D) for k = D..R-1,
+    // then the remaining emits; returns the group, *emit tells the phase's kind
+    auto phase = [&](int64_t i, bool *emit) -> int64_t {
+        if (i < Dp) {
+            *emit = false;
+            return i * W + w;
+        }
+        const int64_t m = i - Dp;
+        if (m < 2 * mid) {
+            *emit = (m & 1) != 0;
+            return (*emit ? m / 2 : D + m / 2) * W + w;
+        }
+        *emit = true;
+        return (mid + (m - 2 * mid)) * W + w;
+    };
+    if (tid == 0)
+        s_abort = 0;
+    __syncthreads();
+    const uint4 rep_v = lds_load16(s_rep);
+    const BfRep rp = bf_rep_parts(rep_v, p.r);
+    if (phases == 0)
+        return;
+    BfStage stage;
+    bool emit = false;
+    int64_t g = phase(0, &emit);
+    if (IMAGE)
+        bf_prefetch<false>(p, g, tid, stage);
+    for (int64_t i = 0; i < phases; ++i) {
+        g = phase(i, &emit);
+        __syncthreads();                       // the previous phase is done with the LDS
+        if (IMAGE) {
+            // register stage: this phase's group, then the next phase's prefetch
+            bf_stage_to_lds(stage, tid, s_in);
+            bool ne = false;
+            const int64_t gn = i + 1 < phases ? phase(i + 1, &ne) : g;
+            bf_prefetch<false>(p, gn, tid, stage);
+        } else {
+            // LDS DMA; the other workgroups on the unit hide the latency
+            load_group<kGroupTiles>(p, g * kGroupTiles, tid, true, s_in);
+        }
+        if (tid == 0)
+            s_halo_last = INT_MIN;
+        if (emit && tid == 0) {
+            // the group's prefix, published by the scanner
+            uint64_t v = ld_agent(a.pref + g);
+            int spins = 0;
+            while ((v & kTagMask) != tag) {
+                if (++spins > a.spin_limit || (spins % 64 == 0 && ld_agent(a.flag) == (tag | 1))) {
+                    st_agent(a.flag, tag | 1);
+                    s_abort = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                v = ld_agent(a.pref + g);
+            }
+            s_kexcl = (int64_t)(v & ~kTagMask);
+        }
+        if (!IMAGE)
+            __builtin_amdgcn_s_waitcnt(0);     // the LDS DMA of the group
+        __syncthreads();
+        if (s_abort)
+            return;
+        if (emit)
+            bf_emit_staged<FULL4, IMAGE>(p, g, s_kexcl, lds, rep_v, rp);
+        else
+            bf_count_staged(p, g, lds, a, tag);
+    }
+}
+
 // ---- batches of independent documents ---------------------------------------
 // The count and emit launches of a batch round run the same per-group code as a
 // single buffer, on a per-document view built from the round's BatchDoc.
@@ -2397,6 +3276,98 @@ hipError_t launch_emit(const ReplaceParams &p, int mode, int table_size, int gt,
     return dispatch_mode(mode, table_size, gt, [&]<int M, int LT, int GT>() {
         return emit_t<M, LT, GT>(p, first_group, groups, s);
     });
+}
+
+hipError_t launch_bf_count(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    if (p.L >= 4)
+        hipLaunchKernelGGL((bf_count_kernel<true>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
+    else
+        hipLaunchKernelGGL((bf_count_kernel<false>), dim3((unsigned)groups), dim3(kThreads), 0, s, p, first_group);
+    return hipGetLastError();
+}
+
+// Workgroups of a persistent launch: the resident capacity of the device for the
+// kernel (compute units x workgroups per unit), looked up once per device.
+template <class K>
+int64_t resident_grid(K kernel)
+{
+    static std::mutex mtx;
+    static std::map<std::pair<int, const void *>, int64_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return 1024;
+    std::lock_guard<std::mutex> lk(mtx);
+    const auto key = std::make_pair(dev, (const void *)kernel);
+    auto it = cache.find(key);
+    if (it != cache.end())
+        return it->second;
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0) != hipSuccess || per <= 0)
+        per = 1;
+    (void)hipGetLastError();
+    const int64_t grid = (int64_t)cus * per;
+    cache[key] = grid;
+    return grid;
+}
+
+// Output staging of the border-free kernels: 1 = an LDS image of each group's output
+// stored by the whole workgroup (with register prefetch), 0 = every thread stores
+// its own pieces (CR_BF_IMAGE, read per launch).
+bool bf_image()
+{
+    const char *e = getenv("CR_BF_IMAGE");
+    return e ? atoi(e) != 0 : true;
+}
+
+template <bool FULL4, bool IMAGE>
+hipError_t sweep_t(const ReplaceParams &p, const SweepArgs &a, hipStream_t s)
+{
+    const int64_t grid = std::min(p.num_groups + 1, resident_grid(bf_sweep_kernel<FULL4, IMAGE>));
+    hipLaunchKernelGGL((bf_sweep_kernel<FULL4, IMAGE>), dim3((unsigned)std::max<int64_t>(grid, 2)), dim3(kThreads),
+                       0, s, p, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, int spin_limit, hipStream_t s)
+{
+    const int64_t G = p.num_groups;
+    SweepArgs a{sweep, sweep + G, sweep + 2 * G, ahead, spin_limit};
+    const bool img = bf_image();
+    if (p.L >= 4)
+        return img ? sweep_t<true, true>(p, a, s) : sweep_t<true, false>(p, a, s);
+    return img ? sweep_t<false, true>(p, a, s) : sweep_t<false, false>(p, a, s);
+}
+
+int64_t sweep_words(int64_t groups) { return 2 * groups + 1; }
+
+hipError_t launch_bf_emit(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s)
+{
+    if (groups <= 0)
+        return hipSuccess;
+    if (!bf_image()) {
+        if (p.L >= 4)
+            hipLaunchKernelGGL((bf_emit_direct_kernel<true>), dim3((unsigned)groups), dim3(kThreads), 0, s, p,
+                               first_group);
+        else
+            hipLaunchKernelGGL((bf_emit_direct_kernel<false>), dim3((unsigned)groups), dim3(kThreads), 0, s, p,
+                               first_group);
+        return hipGetLastError();
+    }
+    if (p.L >= 4) {
+        const int64_t grid = std::min(groups, resident_grid(bf_emit_kernel<true>));
+        hipLaunchKernelGGL((bf_emit_kernel<true>), dim3((unsigned)grid), dim3(kThreads), 0, s, p, first_group,
+                           groups);
+    } else {
+        const int64_t grid = std::min(groups, resident_grid(bf_emit_kernel<false>));
+        hipLaunchKernelGGL((bf_emit_kernel<false>), dim3((unsigned)grid), dim3(kThreads), 0, s, p, first_group,
+                           groups);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s)

@@ -158,6 +158,8 @@ struct Session {
     size_t info_tiles = 0;
     uint32_t *d_bitmap = nullptr;
     size_t bitmap_words = 0;
+    uint64_t *d_sweep = nullptr;     // single-pass sweep: per-group counts and prefixes, abort word
+    size_t sweep_words = 0;
     uint8_t *d_patrep = nullptr;
     size_t patrep_cap = 0;
     uint8_t *h_patrep = nullptr;     // pinned staging for pattern + replacement
@@ -313,6 +315,10 @@ hipError_t next_epoch(Session *s)
                                        sizeof(LaunchStats) / sizeof(uint64_t), s->stream);
         if (e != hipSuccess)
             return e;
+        // the sweep's tagged words too, so no word of an earlier round of epochs matches
+        e = launch_fill_u64(s->d_sweep, 0, s->sweep_words, s->stream);
+        if (e != hipSuccess)
+            return e;
         s->epoch = 0;
     }
     ++s->epoch;
@@ -329,6 +335,46 @@ int64_t chunk_groups()
         return (int64_t)(g > 0 ? g : INT64_MAX / 4);
     }();
     return v;
+}
+
+// Single-pass sweep for border-free passes of at least CR_SWEEP_MIN bytes (default
+// 64 MiB; CR_SWEEP=0 turns it off); CR_SWEEP_AHEAD rounds of count-ahead (default 2).
+// Read at every pass, so a test can force either path.
+int64_t sweep_min_bytes()
+{
+    const char *off = getenv("CR_SWEEP");
+    if (off && atoi(off) == 0)
+        return INT64_MAX;
+    const char *e = getenv("CR_SWEEP_MIN");
+    return e ? (int64_t)atoll(e) : (64ll << 20);
+}
+
+int sweep_ahead()
+{
+    const char *e = getenv("CR_SWEEP_AHEAD");
+    const int v = e ? atoi(e) : 2;
+    return v < 0 ? 0 : (v > 64 ? 64 : v);
+}
+
+int sweep_spin_limit()
+{
+    const char *e = getenv("CR_SWEEP_SPIN");
+    const int v = e ? atoi(e) : 200000;
+    return v < 1 ? 1 : v;
+}
+
+// The sweep's words, cleared when (re)allocated.
+hipError_t ensure_sweep(Session *s, int64_t groups)
+{
+    const size_t need = (size_t)sweep_words(groups);
+    if (s->sweep_words >= need && s->d_sweep)
+        return hipSuccess;
+    size_t cap_bytes = s->sweep_words * 8;
+    hipError_t e = grow((void **)&s->d_sweep, &cap_bytes, need * 8, false, s->stream);
+    if (e != hipSuccess)
+        return e;
+    s->sweep_words = cap_bytes / 8;
+    return launch_fill_u64(s->d_sweep, 0, s->sweep_words, s->stream);
 }
 
 hipError_t ensure_scratch(Session *s, int64_t num_tiles, int mode, int table_size)
@@ -496,7 +542,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     CK(upload_pattern(s, pat, L, rep, r));
     const int64_t num_tiles = (n + kTile - 1) / kTile;
     bool tune = false;
-    const int gt = choose_group_tiles(s->device, mode, table_size, n, &tune);
+    // border-free patterns run the lean pass, whose group is fixed at 16 KiB
+    const int gt = mode == kModeBorderFree ? kGroupTiles : choose_group_tiles(s->device, mode, table_size, n, &tune);
     const int64_t num_groups = groups_of(num_tiles, gt);
     const int64_t cg = chunk_groups();
     const int64_t num_chunks = (num_groups + cg - 1) / cg;
@@ -517,6 +564,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     }
     p.pat4 = pat4;
     p.mask4 = mask4;
+    pattern_qsad(pat, L, p.qref, &p.qx);
     p.stats = s->d_stats;
     p.num_tiles = num_tiles;
     p.num_groups = num_groups;
@@ -551,7 +599,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     p.chunks_pre = s->d_pre;
     p.info = s->d_info;
     p.bitmap = s->d_bitmap;
-    p.masks = mode != kModeBitmap && s->bitmap_words >= (size_t)num_tiles * (kTile / 32) ? s->d_bitmap : nullptr;
+    p.masks = mode == kModeTable && s->bitmap_words >= (size_t)num_tiles * (kTile / 32) ? s->d_bitmap : nullptr;
 
     if (mode == kModeBitmap) {
         // candidates -> serial greedy (gives the exact count) -> emit
@@ -587,7 +635,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             for (int64_t c = 0; c < num_chunks; ++c) {
                 const int64_t g0 = c * cg;
                 const int64_t gn = num_groups - g0 < cg ? num_groups - g0 : cg;
-                hipError_t e = launch_count(p, mode, table_size, gt, g0, gn, s->stream);
+                hipError_t e = mode == kModeBorderFree ? launch_bf_count(p, g0, gn, s->stream)
+                                                       : launch_count(p, mode, table_size, gt, g0, gn, s->stream);
                 if (e != hipSuccess)
                     return e;
                 e = num_chunks == 1 ? launch_spine_blocks(p, mode, table_size, gn, s->stream)
@@ -595,7 +644,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                 if (e != hipSuccess)
                     return e;
                 if (emit) {
-                    e = launch_emit(p, mode, table_size, gt, g0, gn, s->stream);
+                    e = mode == kModeBorderFree ? launch_bf_emit(p, g0, gn, s->stream)
+                                                : launch_emit(p, mode, table_size, gt, g0, gn, s->stream);
                     if (e != hipSuccess)
                         return e;
                 }
@@ -636,18 +686,45 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             return rc;
         }
         p.dst = dst;
-        if (counted) {
-            CK(launch_emit(p, mode, table_size, gt, 0, num_groups, s->stream));
-        } else {
+        // border-free passes of a single chunk: one persistent launch when large enough
+        bool swept = false;
+        if (!counted && mode == kModeBorderFree && num_chunks == 1 && n >= sweep_min_bytes() &&
+            ensure_sweep(s, num_groups) == hipSuccess) {
             if (kernel_ms)
                 CK(hipEventRecord(s->ev0, s->stream));
-            CK(run_chunks(true));
+            CK(launch_bf_sweep(p, s->d_sweep, sweep_ahead(), sweep_spin_limit(), s->stream));
+            if (kernel_ms)
+                CK(hipEventRecord(s->ev1, s->stream));
+            CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * num_groups, 8, hipMemcpyDeviceToHost, s->stream));
+            if (want_last)
+                CK(launch_last_kept(p, s->d_query, s->stream));
+            rc = read_stats(&kept, nullptr);   // synchronizes
+            const uint64_t fv = *(const uint64_t *)(s->h_misc + 96);
+            swept = fv != (((uint64_t)s->epoch << kEpochShift) | 1u);
+            if (swept && rc != 0)
+                return rc;
+            if (!swept) {
+                // a wait gave up (the grid was not all resident): the three launches,
+                // under a new epoch so nothing of the abandoned pass is read
+                CK(next_epoch(s));
+                p.epoch = s->epoch;
+            }
         }
-        if (kernel_ms)
-            CK(hipEventRecord(s->ev1, s->stream));
-        if (want_last)
-            CK(launch_last_kept(p, s->d_query, s->stream));
-        rc = read_stats(&kept, nullptr);
+        if (!swept) {
+            if (counted) {
+                CK(mode == kModeBorderFree ? launch_bf_emit(p, 0, num_groups, s->stream)
+                                           : launch_emit(p, mode, table_size, gt, 0, num_groups, s->stream));
+            } else {
+                if (kernel_ms)
+                    CK(hipEventRecord(s->ev0, s->stream));
+                CK(run_chunks(true));
+            }
+            if (kernel_ms)
+                CK(hipEventRecord(s->ev1, s->stream));
+            if (want_last)
+                CK(launch_last_kept(p, s->d_query, s->stream));
+            rc = read_stats(&kept, nullptr);
+        }
         if (rc != 0)
             return rc;
         if (want_last) {
@@ -1299,7 +1376,8 @@ void destroy(Session *s)
         if (cs)
             (void)hipStreamSynchronize(cs);
     void *dev[] = {s->d_buf, s->d_tmp, s->d_gsum, s->d_chunks, s->d_pre, s->d_info, s->d_bitmap, s->d_patrep,
-                   s->d_presence, s->d_misc, s->d_sin[0], s->d_sin[1], s->d_sout[0], s->d_sout[1], s->d_blocks};
+                   s->d_presence, s->d_misc, s->d_sin[0], s->d_sin[1], s->d_sout[0], s->d_sout[1], s->d_blocks,
+                   s->d_sweep};
     for (void *p : dev)
         if (p)
             (void)hipFree(p);

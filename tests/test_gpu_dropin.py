@@ -223,23 +223,21 @@ def test_device_pass_drops_the_query_record(lib):
     """cr_replace_device runs on the session's scratch: a prefix query after it
     answers 0 / -1 (no stale pattern or freed buffer is read) and the session's
     own result is untouched; the next apply restores the query."""
-    import torch
     from cuda_replace_amd import Session
+    from tests.hipmem import DevBuf
     src = text(100000, 6)
     with Session(lib, src) as s:
         s.apply(b"hello", b"HI")
         want = src.replace(b"hello", b"HI")
         assert s.query_prefix(50000) == ro.query_prefix(src, b"hello", None, 50000)
         big = text(3 << 20, 7)
-        d_src = torch.frombuffer(bytearray(big), dtype=torch.uint8).cuda()
-        d_dst = torch.empty(len(big) * 3 + 16, dtype=torch.uint8, device="cuda")
-        out_len = C.c_size_t()
-        rc = lib.lib.cr_replace_device(s.h, C.c_void_p(d_src.data_ptr()), len(big), C.c_void_p(d_dst.data_ptr()),
-                                       d_dst.numel(), C.cast(C.c_char_p(b"a"), u8p), 1,
-                                       C.cast(C.c_char_p(b"<A>"), u8p), 3, -1, C.byref(out_len), None, None)
-        assert rc == 0
-        torch.cuda.synchronize()
-        assert d_dst[:out_len.value].cpu().numpy().tobytes() == big.replace(b"a", b"<A>")
+        with DevBuf.of(big) as d_src, DevBuf(len(big) * 3 + 16) as d_dst:
+            out_len = C.c_size_t()
+            rc = lib.lib.cr_replace_device(s.h, d_src.ptr, len(big), d_dst.ptr, d_dst.n,
+                                           C.cast(C.c_char_p(b"a"), u8p), 1, C.cast(C.c_char_p(b"<A>"), u8p), 3,
+                                           -1, C.byref(out_len), None, None)
+            assert rc == 0
+            assert d_dst.download(out_len.value) == big.replace(b"a", b"<A>")
         assert s.query_prefix(50000) == (0, -1)
         assert s.result() == want
         s.apply(b"beta", b"")
