@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libcuda_replace.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["cr_kernels.hip", "cr_runtime.hip", "cr_batch.hip", "cr_server.hip"]
+SOURCES = ["cr_kernels.hip", "cr_runtime.hip", "cr_batch.hip", "cr_server.hip", "cr_noalloc.hip"]
 ARCH = os.environ.get("CR_OFFLOAD_ARCH", "gfx950")
 
 

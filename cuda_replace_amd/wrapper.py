@@ -45,7 +45,7 @@ REQUIRED_SYMBOLS = (
     "cr_batch_create", "cr_batch_run", "cr_batch_destroy", "cr_stream_replace",
     "cr_server_create", "cr_server_submit", "cr_server_wait", "cr_server_release", "cr_server_destroy",
     "cr_apply_batch_ex", "cr_batch_fusable", "cr_autotune_set", "cr_autotune_reset", "cr_autotune_report",
-    "cr_batch_set_mode", "cr_batch_info",
+    "cr_batch_set_mode", "cr_batch_info", "cr_replace_device_noalloc",
 )
 
 
@@ -181,6 +181,9 @@ class CudaReplaceLib:
                                           C.c_int, c_uint8_p, C.c_int, C.c_longlong, c_size_t_p,
                                           c_longlong_p, C.POINTER(C.c_float)]
         lib.cr_replace_device.restype = C.c_int
+        lib.cr_replace_device_noalloc.argtypes = [c_void_p, c_size_t, c_void_p, c_size_t, c_uint8_p, C.c_int,
+                                                  c_uint8_p, C.c_int, C.c_longlong, c_void_p, c_size_t_p]
+        lib.cr_replace_device_noalloc.restype = C.c_int
         lib.cr_pattern_mode.argtypes = [c_uint8_p, C.c_int]
         lib.cr_pattern_mode.restype = C.c_int
         lib.cr_batch_create.argtypes = [C.POINTER(c_void_p)]

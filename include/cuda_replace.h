@@ -13,7 +13,8 @@
  * unchanged there).  Offsets and lengths are 64-bit throughout.
  *
  * Return codes: 0 ok; -1 null handle or argument; -2 session allocation failed
- * (open); -3 closed handle; -4 a result does not fit the ABI's int; any
+ * (open); -3 closed handle; -4 a result does not fit the ABI's int; -5 beyond
+ * the limits of cr_replace_device_noalloc; any
  * positive value is a hipError_t (hipErrorInvalidValue=1, hipErrorOutOfMemory=2
  * and hipErrorUnknown=999 equal the CUDA codes the reference returns).
  *
@@ -161,6 +162,22 @@ CR_API int cr_replace_device(void *h, const uint8_t *d_src, size_t n, uint8_t *d
                       size_t dst_cap, const uint8_t *pat, int pat_len,
                       const uint8_t *rep, int rep_len, long long seed,
                       size_t *out_len, long long *out_last, float *kernel_ms);
+
+/* No counterpart in the reference.  For callers that cannot allocate (tight
+ * serving loops, harnesses that forbid hipMalloc): the pass of cr_replace_device
+ * with no session and no allocation at all.  Its scratch is one static device
+ * array of about 20 MiB per GPU (held from library load until the process ends);
+ * it runs on `stream` (a hipStream_t; NULL is the default stream) and returns
+ * when the output is written.  Limits: n <= 4 GiB, pat_len and rep_len <= 256,
+ * and a border-free or carry-table pattern (a self-overlapping pattern longer
+ * than 16 bytes needs the bitmap path); anything beyond returns -5 and touches
+ * nothing.  Every call on one GPU shares the array, so calls that overlap on one
+ * GPU (several threads or streams at once) race.  Arguments and the too-small
+ * dst_cap rule as cr_replace_device. */
+CR_API int cr_replace_device_noalloc(const uint8_t *d_src, size_t n, uint8_t *d_dst,
+                      size_t dst_cap, const uint8_t *pat, int pat_len,
+                      const uint8_t *rep, int rep_len, long long seed,
+                      void *stream, size_t *out_len);
 
 /* Which suppression path a pattern takes: 1 border-free single pass, 2 carry
  * tables (self-overlapping, length <= 16), 3 bitmap + serial greedy, 0 empty. */
