@@ -885,6 +885,105 @@ __global__ void __launch_bounds__(kWave) spine_top_kernel(ReplaceParams p, int64
     }
 }
 
+// Carry-table spine of a pass with at most kSpineThreads groups, one launch of
+// 64 * ceil(groups / 64) threads: lane = group.  Each wave runs one ordered
+// inclusive scan of its groups' tables, with only as many steps as it has groups;
+// wave 0 then scans the waves' totals (as many steps as there are waves) and hands
+// each wave its entering state; a lane's exclusive state is its left neighbour's
+// inclusive table evaluated at that state.  A small document's pass (a few groups)
+// costs a few table compositions instead of a full 1024-group walk.
+template <int LT>
+__global__ void __launch_bounds__(kSpineThreads) spine_small_kernel(ReplaceParams p, int64_t groups)
+{
+    __shared__ uint64_t s_wt[kSpineWaves][LT];
+    __shared__ int64_t s_k[kSpineWaves];
+    __shared__ uint32_t s_c[kSpineWaves];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int nwaves = (int)(blockDim.x / kWave);
+    const int64_t gi = tid;
+    Table<LT> t = tidentity<LT>();
+    if (gi < groups) {
+#pragma unroll
+        for (int c = 0; c < LT; ++c)
+            t.e[c] = p.gsum[gi * LT + c];
+    }
+    const int64_t left = groups - (int64_t)wave * kWave;
+    const int act = left < kWave ? (int)left : kWave;   // >= 1: the launch covers `groups`
+    Table64<LT> inc = widen<LT>(t);
+    for (int d = 1; d < act; d <<= 1) {
+        Table64<LT> nb;
+#pragma unroll
+        for (int c = 0; c < LT; ++c)
+            nb.e[c] = __shfl_up(inc.e[c], d, kWave);
+        if (lane >= d)
+            inc = tcompose64<LT>(nb, inc);
+    }
+    if (lane == act - 1) {
+#pragma unroll
+        for (int c = 0; c < LT; ++c)
+            s_wt[wave][c] = inc.e[c];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        Table64<LT> w = tidentity64<LT>();
+        if (lane < nwaves) {
+#pragma unroll
+            for (int c = 0; c < LT; ++c)
+                w.e[c] = s_wt[lane][c];
+        }
+        for (int d = 1; d < nwaves; d <<= 1) {
+            Table64<LT> nb;
+#pragma unroll
+            for (int c = 0; c < LT; ++c)
+                nb.e[c] = __shfl_up(w.e[c], d, kWave);
+            if (lane >= d)
+                w = tcompose64<LT>(nb, w);
+        }
+        // the pass enters with no kept start and carry 0: wave w starts where the
+        // inclusive scan of waves 0..w-1 leaves carry 0
+        uint64_t before = 0;
+#pragma unroll
+        for (int c = 0; c < LT; ++c) {
+            const uint64_t u = __shfl_up(w.e[c], 1, kWave);
+            if (c == 0 && lane > 0)
+                before = u;
+        }
+        if (lane < nwaves) {
+            s_k[lane] = (int64_t)(before & kCountMask);
+            s_c[lane] = (uint32_t)(before >> kCarryShift);
+        }
+        if (lane == nwaves - 1) {
+            const uint64_t tot = w.e[0];
+            ChunkState cs;
+            cs.k_base = (int64_t)(tot & kCountMask);
+            cs.carry = (int)(tot >> kCarryShift);
+            cs.pad = 0;
+            p.chunks[1] = cs;
+            p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) |
+                               ((uint64_t)cs.k_base & ((1ull << kEpochShift) - 1));
+        }
+    }
+    __syncthreads();
+    const int64_t kw = s_k[wave];
+    const uint32_t cw = s_c[wave];
+    uint64_t e = (uint64_t)cw << kCarryShift;   // lane 0: nothing of this wave before it
+#pragma unroll
+    for (int c = 0; c < LT; ++c) {
+        const uint64_t u = __shfl_up(inc.e[c], 1, kWave);
+        if (lane > 0 && (uint32_t)c == cw)
+            e = u;
+    }
+    if (gi < groups) {
+        ChunkState cs;
+        cs.k_base = kw + (int64_t)(e & kCountMask);
+        cs.carry = (int)(e >> kCarryShift);
+        cs.pad = 0;
+        p.chunks_pre[gi] = cs;
+    }
+}
+
 // ---- emit: resolve kept starts and write each tile's output ----------------
 // The group's kept prefix and carry come from the spine's record for the group.
 // The bitmap mode instead reads each tile's prefix and carry from the records of
@@ -3203,6 +3302,13 @@ hipError_t spine_blocks_t(const ReplaceParams &p, int64_t groups, hipStream_t s)
 {
     constexpr int64_t kPass = spine_block_groups<MODE, LT>();
     const int64_t nb = (groups + kPass - 1) / kPass;
+    if constexpr (MODE == kModeTable) {
+        if (groups <= kSpineThreads) {
+            const unsigned threads = (unsigned)((groups + kWave - 1) / kWave * kWave);
+            hipLaunchKernelGGL((spine_small_kernel<LT>), dim3(1), dim3(threads), 0, s, p, groups);
+            return hipGetLastError();
+        }
+    }
     if (nb <= 1)   // one block: the single-workgroup spine is the whole job
         return spine_t<MODE, LT>(p, 0, groups, 0, 1, s);
     hipLaunchKernelGGL((spine_reduce_kernel<MODE, LT>), dim3((unsigned)nb), dim3(kSpineThreads), 0, s, p, groups);
