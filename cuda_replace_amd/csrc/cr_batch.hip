@@ -241,7 +241,10 @@ int run_other(Engine *E, const DocState &ds, const uint8_t *pat, int L, const ui
     const int64_t groups = (tiles + kGroupTiles - 1) / kGroupTiles;
     const size_t bitmap_bytes = align_up((size_t)(tiles > 0 ? tiles : 1) * (kTile / 8), 256);
     const size_t info_bytes = align_up((size_t)(tiles + 1) * sizeof(TileInfo), 256);
-    const size_t need = pr + 256 + (L > 0 ? bitmap_bytes + info_bytes : 0);
+    // scratch of the parallel kept-start selection (0: the serial greedy runs)
+    const bool bf = L > 0 && border_free(pat, L);
+    const size_t sel_bytes = L > 0 && tiles > 0 ? align_up(bitmap_select_bytes(tiles, L, bf), 256) : 0;
+    const size_t need = pr + 256 + (L > 0 ? bitmap_bytes + info_bytes + sel_bytes : 0);
     CK(dev_reserve(E->d_other, need, false, 0, s));
     CK(host_reserve(E->h_misc, pr + 256));
     if (L > 0)
@@ -288,8 +291,12 @@ int run_other(Engine *E, const DocState &ds, const uint8_t *pat, int L, const ui
     p.bitmap = reinterpret_cast<uint32_t *>(E->d_other.p + pr + 256);
     p.info = reinterpret_cast<TileInfo *>(E->d_other.p + pr + 256 + bitmap_bytes);
     CK(launch_fill_u64(reinterpret_cast<uint64_t *>(p.stats), 0, 4, s));
-    CK(launch_bitmap_mark(p, s));
-    CK(launch_bitmap_greedy(p, s));
+    if (sel_bytes) {
+        CK(launch_bitmap_select(p, bf, E->d_other.p + pr + 256 + bitmap_bytes + info_bytes, s));
+    } else {
+        CK(launch_bitmap_mark(p, s));
+        CK(launch_bitmap_greedy(p, s));
+    }
     CK(hipMemcpyAsync(E->h_misc.p + pr, p.stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
     const uint64_t kt = reinterpret_cast<const LaunchStats *>(E->h_misc.p + pr)->k_total;

@@ -297,6 +297,13 @@ hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, i
 int64_t sweep_words(int64_t groups);
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s);
+// Parallel bitmap selection (candidates, then per-tile tables composed level by level,
+// then every tile resolved): bytes of scratch it needs for a pass, 0 when the serial
+// greedy must run (a self-overlapping pattern over 256 bytes).  It also marks the
+// candidates, and leaves the kept bitmap, the tile records and the stats as the
+// serial greedy does.
+size_t bitmap_select_bytes(int64_t num_tiles, int L, bool border_free);
+hipError_t launch_bitmap_select(const ReplaceParams &p, bool border_free, uint8_t *scratch, hipStream_t s);
 hipError_t launch_empty_pattern(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *ptr, uint64_t value, size_t count, hipStream_t s);
 hipError_t launch_query(const ReplaceParams &p, int mode, int64_t limit,

@@ -2961,16 +2961,44 @@ mark_bitmap_kernel(ReplaceParams p, int64_t words)
         return;
     const int64_t base = wi * 32;
     const int L = p.L;
-    const uint8_t p0 = p.pat[0];
+    // the 32 positions' first four bytes from nine dword loads; the dword that holds
+    // the end of the buffer is read byte by byte, so nothing past n is touched
+    uint32_t d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int64_t o = base + 4 * k;
+        if (o + 4 <= p.n) {
+            __builtin_memcpy(&d[k], p.src + o, 4);
+        } else {
+            uint32_t v = 0;
+            for (int b = 0; b < 4 && o + b < p.n; ++b)
+                v |= (uint32_t)p.src[o + b] << (8 * b);
+            d[k] = v;
+        }
+    }
     uint32_t m = 0;
-    for (int j = 0; j < 32; ++j) {
+#pragma unroll
+    for (int j = 31; j >= 0; --j) {
+        const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
+        m = m + m + (uint32_t)(((w ^ p.pat4) & p.mask4) == 0u);
+    }
+    const int64_t lo = p.floor - base;
+    const int64_t hi = p.n - (int64_t)L - base;
+    uint32_t rm = 0;
+    if (hi >= 0 && lo <= 31 && lo <= hi) {
+        const int a = lo < 0 ? 0 : (int)lo;
+        const int b = hi > 31 ? 31 : (int)hi;
+        rm = (b == 31 ? 0xffffffffu : ((1u << (b + 1)) - 1u)) & (0xffffffffu << a);
+    }
+    m &= rm;
+    uint32_t mm = m;
+    m = 0;
+    while (mm) {
+        const int j = __builtin_ctz(mm);
+        mm &= mm - 1;
         const int64_t pos = base + j;
-        if (pos < p.floor || pos + L > p.n)
-            continue;
-        if (p.src[pos] != p0)
-            continue;
         bool ok = true;
-        for (int k = 1; k < L; ++k) {
+        for (int k = 4; k < L; ++k) {
             if (p.src[pos + k] != p.pat[k]) {
                 ok = false;
                 break;
@@ -3052,6 +3080,361 @@ greedy_bitmap_kernel(ReplaceParams p, int64_t words)
         p.stats->k_total = ((uint64_t)p.epoch << kEpochShift) | (uint64_t)k;
         p.stats->last_kept = ((uint64_t)p.epoch << kEpochShift) | (uint64_t)(last + 1);
     }
+}
+
+// ---- bitmap mode: parallel leftmost non-overlapping selection ----------------
+// The serial greedy above walks every candidate in one wave.  The parallel form
+// (cuda_replace.cu:905-968 is the serial rule it reproduces) works on tiles:
+//   tables   for every entry state s (the tile's first s bytes are covered by an
+//            earlier kept match, s < L) the greedy over the tile's candidates gives
+//            (carry out, kept count); L states, one walk each;
+//   compose  64 consecutive tables at a time into their composition, level by
+//            level, until at most 64 remain;
+//   top      one thread walks the last level from state 0: each entry's (kept
+//            before, entry state);
+//   expand   each parent's entry walked over its 64 children, level by level
+//            down to the tiles;
+//   resolve  every tile runs its greedy from its entry state, rewrites its bitmap
+//            words to the kept starts and writes its tile record.
+// A border-free pattern keeps every candidate, so its tile tables have one state
+// (the tile's candidate count), and a tile's carry in comes from the last
+// candidate of the tiles before it.
+constexpr int kBmFan = 64;
+
+// A tile's candidate words in LDS and, for each word index, the first nonzero word
+// at or after it (kTile / 32 when none).
+struct BmTile {
+    uint32_t w[kTile / 32];
+    uint16_t nnz[kTile / 32 + 1];
+};
+
+// Loads tile t's candidate words (zero past the buffer) and builds nnz; every thread
+// of the block takes part (blockDim.x divides 256).
+__device__ __forceinline__ void bm_load_tile(const ReplaceParams &p, int64_t t, BmTile &s)
+{
+    constexpr int W = kTile / 32;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int64_t words = p.num_tiles * W;
+    for (int i = tid; i < W; i += bd) {
+        const int64_t wi = t * W + i;
+        const uint32_t v = wi < words ? p.bitmap[wi] : 0u;
+        s.w[i] = v;
+        s.nnz[i] = v ? (uint16_t)i : (uint16_t)W;
+    }
+    if (tid == 0)
+        s.nnz[W] = (uint16_t)W;
+    __syncthreads();
+    // suffix minimum over the word indices, doubling the reach each round
+    for (int d = 1; d < W; d <<= 1) {
+        uint16_t v[W / 64];
+        int k = 0;
+        for (int i = tid; i < W; i += bd, ++k) {
+            const uint16_t a = s.nnz[i];
+            const uint16_t b = i + d < W ? s.nnz[i + d] : (uint16_t)W;
+            v[k] = a < b ? a : b;
+        }
+        __syncthreads();
+        k = 0;
+        for (int i = tid; i < W; i += bd, ++k)
+            s.nnz[i] = v[k];
+        __syncthreads();
+    }
+}
+
+// First candidate at or after tile offset x, or -1.
+__device__ __forceinline__ int bm_first(const BmTile &s, int x)
+{
+    constexpr int W = kTile / 32;
+    if (x >= kTile)
+        return -1;
+    const int wi = x >> 5;
+    const uint32_t m = s.w[wi] & (0xffffffffu << (x & 31));
+    if (m)
+        return (wi << 5) | __builtin_ctz(m);
+    const int j = s.nnz[wi + 1];
+    return j >= W ? -1 : (j << 5) | __builtin_ctz(s.w[j]);
+}
+
+// The greedy over tile offsets [x, end) from entry offset x: kept count, carry past
+// end, last kept; with MARK each kept start's bit is set in kept[].
+template <bool MARK>
+__device__ __forceinline__ void bm_walk(const BmTile &s, int x, int end, int L, int *count, int *carry, int *last,
+                                        uint32_t *kept)
+{
+    int cnt = 0, lst = -1;
+    for (;;) {
+        const int c = bm_first(s, x);
+        if (c < 0 || c >= end)
+            break;
+        if (MARK)
+            kept[c >> 5] |= 1u << (c & 31);
+        ++cnt;
+        lst = c;
+        x = c + L;
+    }
+    *count = cnt;
+    *carry = x > end ? x - end : 0;
+    *last = lst;
+}
+
+// Candidates of tile t from LDS (L <= kHalo), one 32-position word per thread: the
+// head word compared at every position as the count kernels do, then pattern word k
+// (bytes 4k..4k+3, masked past L) at every position, while any lane of the wave
+// still has a candidate.  Dense inputs (every position a candidate) cost
+// ceil(L / 4) word compares per position instead of L byte compares.
+__global__ void __launch_bounds__(kThreads) bm_mark_lds_kernel(ReplaceParams p)
+{
+    __shared__ __align__(16) uint8_t s_in[GroupShape<1>::kIn];
+    __shared__ __align__(16) uint8_t s_pat[kMaxSinglePassPat + 4];
+    const int64_t t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int L = p.L;
+    for (int i = tid; i < kMaxSinglePassPat + 4; i += kThreads)
+        s_pat[i] = i < L ? p.pat[i] : 0;
+    load_group<1>(p, t, tid, false, s_in);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int seg = tid * 32;
+    const int64_t gseg = t * kTile + (int64_t)seg;
+    uint32_t m = 0;
+    if (gseg < p.n) {
+        const uint8_t *s_tile = s_in + kHalo;
+        const int64_t lo = p.floor - gseg;
+        const int64_t hi = p.n - (int64_t)L - gseg;
+        if (hi >= 0 && lo <= 31 && lo <= hi) {
+            const int a = lo < 0 ? 0 : (int)lo;
+            const int b = hi > 31 ? 31 : (int)hi;
+            m = (b == 31 ? 0xffffffffu : ((1u << (b + 1)) - 1u)) & (0xffffffffu << a);
+        }
+    }
+    const int words = (L + 3) / 4;
+    for (int k = 0; k < words; ++k) {
+        if (__builtin_amdgcn_ballot_w64(m != 0u) == 0)
+            break;
+        const uint32_t pw = *reinterpret_cast<const uint32_t *>(s_pat + 4 * k);
+        const int left = L - 4 * k;
+        const uint32_t pm = left >= 4 ? 0xffffffffu : (0xffffffffu >> (8 * (4 - left)));
+        const uint32_t *w32 = reinterpret_cast<const uint32_t *>(s_in + kHalo + seg + 4 * k);
+        uint32_t d[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            d[q] = w32[q];
+        uint32_t mk = 0;
+#pragma unroll
+        for (int j = 31; j >= 0; --j) {
+            const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
+            mk = mk + mk + (uint32_t)(((w ^ pw) & pm) == 0u);
+        }
+        m &= mk;
+    }
+    if (gseg < p.n)
+        p.bitmap[t * (kTile / 32) + tid] = m;
+    else
+        p.bitmap[t * (kTile / 32) + tid] = 0u;
+}
+
+// Level-0 tables, one per leaf (a tile holds lpt leaves of kTile / lpt bytes): entry s
+// of a leaf = carry_out << 16 | kept (kept <= 8192, carry < L <= 256) for the greedy
+// over the leaf whose first s bytes are covered.  Border-free (one state, lpt = 1):
+// the tile's candidate count, and its last candidate (tile offset, or -1) in tlast.
+__global__ void __launch_bounds__(kThreads) bm_tables_kernel(ReplaceParams p, uint32_t *tab, int states, int lpt,
+                                                              int32_t *tlast)
+{
+    __shared__ BmTile s;
+    const int64_t t = blockIdx.x;
+    bm_load_tile(p, t, s);
+    const int tid = threadIdx.x;
+    if (tlast) {
+        // border-free: every candidate is kept
+        const uint32_t v = s.w[tid];
+        uint32_t c = (uint32_t)__builtin_popcount(v);
+        int l = v ? tid * 32 + 31 - __builtin_clz(v) : -1;
+        c = wave_sum_u32(c);
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1)
+            l = max(l, __shfl_xor(l, d, kWave));
+        __shared__ uint32_t s_c[kWaves];
+        __shared__ int s_l[kWaves];
+        if ((tid & (kWave - 1)) == 0) {
+            s_c[tid / kWave] = c;
+            s_l[tid / kWave] = l;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            tab[t] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+            tlast[t] = max(max(s_l[0], s_l[1]), max(s_l[2], s_l[3]));
+        }
+        return;
+    }
+    const int leaf_bytes = kTile / lpt;
+    for (int i = tid; i < lpt * states; i += blockDim.x) {
+        const int leaf = i / states, st = i - leaf * states;
+        int cnt, carry, lst;
+        bm_walk<false>(s, leaf * leaf_bytes + st, (leaf + 1) * leaf_bytes, p.L, &cnt, &carry, &lst, nullptr);
+        tab[(t * lpt + leaf) * states + st] = ((uint32_t)carry << 16) | (uint32_t)cnt;
+    }
+}
+
+// Entry e of a table at a level: level 0 packs carry << 16 | count in 32 bits,
+// higher levels carry << 32 | count in 64.
+__device__ __forceinline__ void bm_entry(const uint32_t *t32, const uint64_t *t64, int64_t idx, uint32_t *carry,
+                                         uint64_t *count)
+{
+    if (t32) {
+        const uint32_t e = t32[idx];
+        *carry = e >> 16;
+        *count = e & 0xffffu;
+    } else {
+        const uint64_t e = t64[idx];
+        *carry = (uint32_t)(e >> 32);
+        *count = e & 0xffffffffull;
+    }
+}
+
+// out[b] = in[64b] then in[64b+1] ... then in[64b+63] (composition), per state.
+__global__ void __launch_bounds__(kThreads) bm_compose_kernel(const uint32_t *in32, const uint64_t *in64,
+                                                               int64_t n_in, int states, uint64_t *out)
+{
+    const int64_t b = blockIdx.x;
+    for (int s0 = threadIdx.x; s0 < states; s0 += blockDim.x) {
+        uint32_t st = (uint32_t)s0;
+        uint64_t cnt = 0;
+        for (int j = 0; j < kBmFan; ++j) {
+            const int64_t i = b * kBmFan + j;
+            if (i >= n_in)
+                break;
+            uint32_t c;
+            uint64_t k;
+            bm_entry(in32, in64, i * states + st, &c, &k);
+            cnt += k;
+            st = c;
+        }
+        out[b * states + s0] = ((uint64_t)st << 32) | cnt;
+    }
+}
+
+// One thread walks the last level from state 0: ent[i] = (kept before, entry state).
+__global__ void bm_top_kernel(const uint32_t *in32, const uint64_t *in64, int64_t n, int states, ChunkState *ent,
+                              LaunchStats *stats, uint32_t epoch)
+{
+    if (threadIdx.x != 0)
+        return;
+    int64_t k = 0;
+    uint32_t st = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        ChunkState cs;
+        cs.k_base = k;
+        cs.carry = (int)st;
+        cs.pad = 0;
+        ent[i] = cs;
+        uint32_t c;
+        uint64_t kk;
+        bm_entry(in32, in64, i * states + st, &c, &kk);
+        k += (int64_t)kk;
+        st = c;
+    }
+    stats->k_total = ((uint64_t)epoch << kEpochShift) | ((uint64_t)k & ((1ull << kEpochShift) - 1));
+}
+
+// Children of parent b get their entries by walking its 64 tables from b's entry.
+__global__ void bm_expand_kernel(const uint32_t *in32, const uint64_t *in64, int64_t n_in, int states,
+                                 const ChunkState *parent, ChunkState *child)
+{
+    if (threadIdx.x != 0)
+        return;
+    const int64_t b = blockIdx.x;
+    int64_t k = parent[b].k_base;
+    uint32_t st = (uint32_t)parent[b].carry;
+    for (int j = 0; j < kBmFan; ++j) {
+        const int64_t i = b * kBmFan + j;
+        if (i >= n_in)
+            break;
+        ChunkState cs;
+        cs.k_base = k;
+        cs.carry = (int)st;
+        cs.pad = 0;
+        child[i] = cs;
+        uint32_t c;
+        uint64_t kk;
+        bm_entry(in32, in64, i * states + st, &c, &kk);
+        k += (int64_t)kk;
+        st = c;
+    }
+}
+
+// Self-overlapping: each leaf of tile t runs its greedy from its entry state (lane
+// = leaf), the tile's bitmap words are rewritten to the kept starts and its record
+// is written.
+__global__ void __launch_bounds__(kWave) bm_resolve_kernel(ReplaceParams p, const ChunkState *ent, int lpt)
+{
+    __shared__ BmTile s;
+    __shared__ uint32_t s_kept[kTile / 32];
+    __shared__ int s_last;
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    bm_load_tile(p, t, s);
+    for (int i = lane; i < kTile / 32; i += kWave)
+        s_kept[i] = 0u;
+    if (lane == 0)
+        s_last = -1;
+    __syncthreads();
+    if (lane < lpt) {
+        const int leaf_bytes = kTile / lpt;
+        const ChunkState e = ent[t * lpt + lane];
+        int cnt, carry, lst;
+        bm_walk<true>(s, lane * leaf_bytes + e.carry, (lane + 1) * leaf_bytes, p.L, &cnt, &carry, &lst, s_kept);
+        if (lst >= 0)
+            atomicMax(&s_last, lst);
+    }
+    __syncthreads();
+    const int64_t words = p.num_tiles * (kTile / 32);
+    for (int i = lane; i < kTile / 32; i += kWave) {
+        const int64_t wi = t * (kTile / 32) + i;
+        if (wi < words)
+            p.bitmap[wi] = s_kept[i];
+    }
+    if (lane == 0) {
+        const ChunkState e = ent[t * lpt];
+        const int64_t tstart = t * (int64_t)kTile;
+        TileInfo ti;
+        ti.k_excl = e.k_base;
+        ti.last_abs = s_last >= 0 ? tstart + s_last : -1;
+        ti.carry_in = e.carry;
+        ti.mode = kModeBitmap;
+        p.info[t] = ti;
+        if (s_last >= 0)
+            atomicMax((unsigned long long *)&p.stats->last_kept,
+                      (unsigned long long)(((uint64_t)p.epoch << kEpochShift) | (uint64_t)(tstart + s_last + 1)));
+    }
+}
+
+// Border-free: tile records from the entries (kept before), the tile's last
+// candidate and the carry from the latest candidate of the tiles before it (a
+// candidate ends at most L - 1 bytes later, so only tiles within L bytes matter).
+__global__ void __launch_bounds__(kThreads) bm_bf_records_kernel(ReplaceParams p, const ChunkState *ent,
+                                                                  const int32_t *tlast)
+{
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= p.num_tiles)
+        return;
+    const int64_t tstart = t * (int64_t)kTile;
+    int64_t carry = 0;
+    for (int64_t u = t - 1; u >= 0 && (u + 1) * (int64_t)kTile + p.L > tstart; --u) {
+        if (tlast[u] >= 0) {
+            const int64_t end = u * (int64_t)kTile + tlast[u] + p.L;
+            carry = end > tstart ? end - tstart : 0;
+            break;
+        }
+    }
+    TileInfo ti;
+    ti.k_excl = ent[t].k_base;
+    ti.last_abs = tlast[t] >= 0 ? tstart + tlast[t] : -1;
+    ti.carry_in = (int32_t)(carry < INT_MAX ? carry : INT_MAX);
+    ti.mode = kModeBitmap;
+    p.info[t] = ti;
+    if (tlast[t] >= 0)
+        atomicMax((unsigned long long *)&p.stats->last_kept,
+                  (unsigned long long)(((uint64_t)p.epoch << kEpochShift) | (uint64_t)(tstart + tlast[t] + 1)));
 }
 
 // ---- empty pattern: bytes.replace(b"", rep) ---------------------------------
@@ -3488,6 +3871,100 @@ hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s)
 {
     const int64_t words = p.num_tiles * (kTile / 32);
     hipLaunchKernelGGL(greedy_bitmap_kernel, dim3(1), dim3(kWave), 0, s, p, words);
+    return hipGetLastError();
+}
+
+namespace {
+// Scratch layout of a parallel bitmap selection: level sizes (level 0 = tiles, each
+// next level 1/64 of the previous, until at most kBmFan tables remain), the tables
+// of every level, the entries of every level, and the border-free last candidates.
+struct BmPlan {
+    int states = 0;
+    int levels = 0;              // tables levels 0..levels-1
+    int64_t n[24] = {};
+    size_t tab_off[24] = {};
+    size_t ent_off[24] = {};
+    size_t tlast_off = 0;
+    size_t bytes = 0;
+};
+
+BmPlan bm_plan(int64_t tiles, int states)
+{
+    BmPlan pl;
+    pl.states = states;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    int64_t cnt = tiles;
+    for (;;) {
+        pl.n[pl.levels] = cnt;
+        pl.tab_off[pl.levels] = take((size_t)cnt * states * (pl.levels == 0 ? 4 : 8));
+        ++pl.levels;
+        if (cnt <= kBmFan)
+            break;
+        cnt = (cnt + kBmFan - 1) / kBmFan;
+    }
+    for (int l = 0; l < pl.levels; ++l)
+        pl.ent_off[l] = take((size_t)pl.n[l] * sizeof(ChunkState));
+    pl.tlast_off = take((size_t)tiles * 4);
+    pl.bytes = off;
+    return pl;
+}
+}   // namespace
+
+// Leaves per tile: 1 KiB leaves for patterns of up to 32 bytes (a leaf's walks are
+// then 1/8 as long and a tile resolves eight leaves at once), else the tile.
+int bm_leaves(int L, bool border_free)
+{
+    return !border_free && L <= 32 ? 8 : 1;
+}
+
+size_t bitmap_select_bytes(int64_t num_tiles, int L, bool border_free)
+{
+    if (!border_free && L > kMaxSinglePassPat)
+        return 0;   // a self-overlapping pattern over 256 bytes keeps the serial greedy
+    return bm_plan(num_tiles * bm_leaves(L, border_free), border_free ? 1 : L).bytes;
+}
+
+hipError_t launch_bitmap_select(const ReplaceParams &p, bool border_free, uint8_t *scratch, hipStream_t s)
+{
+    const int64_t tiles = p.num_tiles;
+    const int lpt = bm_leaves(p.L, border_free);
+    const BmPlan pl = bm_plan(tiles * lpt, border_free ? 1 : p.L);
+    const int st = pl.states;
+    auto t32 = [&](int l) { return l == 0 ? reinterpret_cast<uint32_t *>(scratch + pl.tab_off[0]) : nullptr; };
+    auto t64 = [&](int l) { return l == 0 ? nullptr : reinterpret_cast<uint64_t *>(scratch + pl.tab_off[l]); };
+    auto ent = [&](int l) { return reinterpret_cast<ChunkState *>(scratch + pl.ent_off[l]); };
+    int32_t *tlast = border_free ? reinterpret_cast<int32_t *>(scratch + pl.tlast_off) : nullptr;
+    if (p.L <= kHalo)
+        hipLaunchKernelGGL(bm_mark_lds_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, s, p);
+    else {
+        const hipError_t e = launch_bitmap_mark(p, s);
+        if (e != hipSuccess)
+            return e;
+    }
+    // one wave per tile when the states fit one (more tiles in flight per unit: the
+    // walks are latency-bound); border-free tiles reduce with the full block
+    const unsigned tab_threads = (!border_free && st * lpt <= kWave) ? (unsigned)kWave : (unsigned)kThreads;
+    hipLaunchKernelGGL(bm_tables_kernel, dim3((unsigned)tiles), dim3(tab_threads), 0, s, p, t32(0), st, lpt, tlast);
+    const unsigned comp_threads = (unsigned)(st >= kThreads ? kThreads : (st + kWave - 1) / kWave * kWave);
+    for (int l = 1; l < pl.levels; ++l)
+        hipLaunchKernelGGL(bm_compose_kernel, dim3((unsigned)pl.n[l]), dim3(comp_threads), 0, s, t32(l - 1),
+                           t64(l - 1), pl.n[l - 1], st, t64(l));
+    const int top = pl.levels - 1;
+    hipLaunchKernelGGL(bm_top_kernel, dim3(1), dim3(1), 0, s, t32(top), t64(top), pl.n[top], st, ent(top), p.stats,
+                       p.epoch);
+    for (int l = top; l >= 1; --l)
+        hipLaunchKernelGGL(bm_expand_kernel, dim3((unsigned)pl.n[l]), dim3(1), 0, s, t32(l - 1), t64(l - 1),
+                           pl.n[l - 1], st, ent(l), ent(l - 1));
+    if (border_free)
+        hipLaunchKernelGGL(bm_bf_records_kernel, dim3((unsigned)((tiles + kThreads - 1) / kThreads)), dim3(kThreads),
+                           0, s, p, ent(0), tlast);
+    else
+        hipLaunchKernelGGL(bm_resolve_kernel, dim3((unsigned)tiles), dim3(kWave), 0, s, p, ent(0), lpt);
     return hipGetLastError();
 }
 

@@ -160,6 +160,8 @@ struct Session {
     size_t bitmap_words = 0;
     uint64_t *d_sweep = nullptr;     // single-pass sweep: per-group counts and prefixes, abort word
     size_t sweep_words = 0;
+    uint8_t *d_bmsel = nullptr;      // parallel bitmap selection: tile tables, levels, entries
+    size_t bmsel_cap = 0;
     uint8_t *d_patrep = nullptr;
     size_t patrep_cap = 0;
     uint8_t *h_patrep = nullptr;     // pinned staging for pattern + replacement
@@ -340,6 +342,14 @@ int64_t chunk_groups()
 // Single-pass sweep for border-free passes of at least CR_SWEEP_MIN bytes (default
 // 64 MiB; CR_SWEEP=0 turns it off); CR_SWEEP_AHEAD rounds of count-ahead (default 2).
 // Read at every pass, so a test can force either path.
+// Bitmap-mode passes select kept starts in parallel (CR_BITMAP_SERIAL=1 forces the
+// one-wave serial greedy, for comparison).  Read at every pass.
+bool bitmap_parallel_enabled()
+{
+    const char *e = getenv("CR_BITMAP_SERIAL");
+    return !(e && atoi(e) != 0);
+}
+
 int64_t sweep_min_bytes()
 {
     const char *off = getenv("CR_SWEEP");
@@ -607,8 +617,25 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         p.epoch = s->epoch;
         if (kernel_ms)
             CK(hipEventRecord(s->ev0, s->stream));
-        CK(launch_bitmap_mark(p, s->stream));
-        CK(launch_bitmap_greedy(p, s->stream));
+        // the parallel selection when its scratch is there, else the serial greedy
+        const bool bf = border_free(pat, L);
+        const size_t sel = bitmap_select_bytes(num_tiles, L, bf);
+        bool parallel = false;
+        if (sel && bitmap_parallel_enabled()) {
+            size_t cap = s->bmsel_cap;
+            if (grow((void **)&s->d_bmsel, &cap, sel, false, s->stream) == hipSuccess) {
+                s->bmsel_cap = cap;
+                parallel = true;
+            } else {
+                (void)hipGetLastError();
+            }
+        }
+        if (parallel) {
+            CK(launch_bitmap_select(p, bf, s->d_bmsel, s->stream));
+        } else {
+            CK(launch_bitmap_mark(p, s->stream));
+            CK(launch_bitmap_greedy(p, s->stream));
+        }
         int rc = read_stats(&kept, &last);
         if (rc != 0)
             return rc;
@@ -1377,7 +1404,7 @@ void destroy(Session *s)
             (void)hipStreamSynchronize(cs);
     void *dev[] = {s->d_buf, s->d_tmp, s->d_gsum, s->d_chunks, s->d_pre, s->d_info, s->d_bitmap, s->d_patrep,
                    s->d_presence, s->d_misc, s->d_sin[0], s->d_sin[1], s->d_sout[0], s->d_sout[1], s->d_blocks,
-                   s->d_sweep};
+                   s->d_sweep, s->d_bmsel};
     for (void *p : dev)
         if (p)
             (void)hipFree(p);
