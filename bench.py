@@ -30,11 +30,12 @@ batched  = the uneven-documents workload (8192 documents, log-normal sizes, 1-16
 serving  = open-loop Poisson arrivals (1000 requests/s for --serve-seconds) of
            requests sliced from the text (log-normal sizes, median 1 MiB, sigma 2,
            4 KiB - 256 MiB; 1-32 pairs each) into the continuous-batching server
-           (4 workers, 200 GB budget): p50 / p99 latency from scheduled arrival to
-           completion, aggregate input GB/s, a sample checked against the oracle
-           after the timed section; beside it the server's capacity on a burst of
-           the same mix and the per-call loop (one cuda_replace_unified per pair) on
-           the same mix, and their ratio.
+           (4 workers, 200 GB budget; requests join a worker's live set between its
+           rounds): p50 / p99 latency from scheduled arrival to completion,
+           aggregate input GB/s, a sample checked against the oracle after the timed
+           section; the same arrivals through the batch-at-a-time form beside it;
+           the server's capacity on a burst of the same mix and the per-call loop
+           (one cuda_replace_unified per pair) on the same mix, and their ratio.
 streaming = the text repeated to --stream-gib GiB, streamed through the pipelined
            chunked path in 256 MiB chunks (fragment -> 12 bytes), host to host,
            checked against bytes.replace of the 1 GiB piece.
@@ -460,12 +461,15 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
     pool = workloads.pair_pool(text[: 4 << 20], 256, seed=8)
     pairs = [[pool[int(k)] for k in rng.integers(0, len(pool), int(c))] for c in rng.integers(1, 33, nreq)]
     check = set(int(i) for i in rng.choice(np.flatnonzero(sizes <= (8 << 20)), size=min(40, nreq), replace=False))
-    lag = np.zeros(nreq)
-    tickets = []
     import gc
-    gc.collect()
-    with Server(lib, hbm_budget=200 << 30, workers=workers) as srv:
+
+    def open_loop(srv):
+        """Submits the arrivals on schedule; latencies, queue times, live-set sizes,
+        wall time, and the sample's results checked after the timed section."""
+        lag = np.zeros(nreq)
+        tickets = []
         srv.submit_at(text, 0, 1 << 20, pairs[0]).result()      # warm the engines
+        gc.collect()
         gc.disable()   # the load generator's own pauses must not delay arrivals
         t0 = time.perf_counter()
         for i in range(nreq):
@@ -490,12 +494,15 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
             else:
                 t.release()
         wall = time.perf_counter() - t0
-        exact = True
+        ok = True
         for i, (t, view) in held.items():
             o = offs[i]
-            exact &= bytes(view) == ro.apply_batch(text[o:o + int(sizes[i])], pairs[i])
+            ok &= bytes(view) == ro.apply_batch(text[o:o + int(sizes[i])], pairs[i])
             t.release()
-        held.clear()
+        return lat, qms, batch_docs, lag, wall, ok
+
+    with Server(lib, hbm_budget=200 << 30, workers=workers) as srv:
+        lat, qms, batch_docs, lag, wall, exact = open_loop(srv)
         # capacity: the first requests of the same mix submitted at once (closed loop)
         sat = min(nreq, 2000)
         t1 = time.perf_counter()
@@ -519,8 +526,18 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
         if time.perf_counter() - t2 > 8.0:
             break
     pc_wall = time.perf_counter() - t2
+    # the same arrivals through the batch-at-a-time form (a free worker takes the
+    # queued requests as one batch and returns them together)
+    os.environ["CR_SERVER_LIVE"] = "0"
+    try:
+        with Server(lib, hbm_budget=200 << 30, workers=workers) as srv:
+            lat_b, qms_b, batch_b, _, wall_b, exact_b = open_loop(srv)
+    finally:
+        os.environ.pop("CR_SERVER_LIVE", None)
+    exact &= exact_b
     log(f"[bench] serving: {nreq} requests in {wall:.2f}s, p50 {np.percentile(lat, 50):.2f} ms, "
-        f"p99 {np.percentile(lat, 99):.2f} ms, bit_exact(sample)={exact}")
+        f"p99 {np.percentile(lat, 99):.2f} ms, bit_exact(sample)={exact}; batch-at-a-time p50 "
+        f"{np.percentile(lat_b, 50):.2f} ms, p99 {np.percentile(lat_b, 99):.2f} ms")
     return {
         "workload": f"open-loop Poisson {rate:.0f} req/s for {seconds:.1f}s; log-normal sizes median 1 MiB, "
                     f"sigma 2, 4 KiB-256 MiB (slices of the text); 1-32 pairs each; {workers} workers, 200 GB budget",
@@ -543,6 +560,15 @@ def serve_bench(lib, ro, text: bytes, seconds: float, rate: float = 1000.0, seed
                      "note": "same request mix in arrival order, one cuda_replace_unified per pair, "
                              "host to host"},
         "served_vs_per_call": round((sat_bytes / sat_wall) / (pc_bytes / pc_wall), 1),
+        "admission": "live rounds: each worker's documents advance one round at a time and queued "
+                     "requests join before every round; mean_batch_docs = documents live on the worker "
+                     "at a request's admission, itself included",
+        "batch_at_a_time": {"p50_ms": round(float(np.percentile(lat_b, 50)), 3),
+                            "p99_ms": round(float(np.percentile(lat_b, 99)), 3),
+                            "mean_batch_docs": round(float(np.mean(batch_b)), 1),
+                            "achieved_GBps": round(float(sizes.sum()) / wall_b / 1e9, 3),
+                            "note": "same arrivals; a free worker takes the queued requests as one batch "
+                                    "(CR_SERVER_LIVE=0)"},
         "bit_exact": bool(exact),
         "checked_sample": len(check),
     }

@@ -316,6 +316,274 @@ int run_other(Engine *E, const DocState &ds, const uint8_t *pat, int L, const ui
     return 0;
 }
 
+// Final slots of finished documents, packed into the engine's final arena.  The
+// arena may move when it grows (its contents are kept); offsets stay valid.
+struct FinArena {
+    Engine *E;
+    size_t used = 0;
+    // room for `bytes` more, made before any address of this round is handed out
+    int reserve(size_t bytes)
+    {
+        if (used + bytes + kDocAlign > E->fin.cap)
+            return (int)dev_reserve(E->fin, used + bytes + kDocAlign, true, used, E->stream);
+        return 0;
+    }
+    int slot(int64_t len, uint8_t **dst, size_t *off)
+    {
+        const size_t need = used + align_up((size_t)len, kDocAlign) + kDocAlign;
+        if (need > E->fin.cap) {
+            const hipError_t e = dev_reserve(E->fin, need, true, used, E->stream);
+            if (e != hipSuccess)
+                return (int)e;
+        }
+        *off = used;
+        *dst = E->fin.p + used;
+        used += align_up((size_t)len, kDocAlign);
+        return 0;
+    }
+};
+
+// One document's step of a round: apply (pat, rep) to its current contents.
+struct RoundItem {
+    DocState *d;
+    const uint8_t *pat;
+    int L;
+    const uint8_t *rep;
+    int r;
+    bool last;            // the document's final pair: its output goes to a final slot
+    int64_t new_len;      // output length (set by the round)
+};
+
+// One host-driven round over the items, each document at its own pair: class-grouped
+// batched counts and spines, one readback of the kept counts, destinations
+// (arena[par ^ 1] for documents that go on, final slots from fin for the
+// others), batched emits, then empty-pattern and bitmap-path documents one at a
+// time.  Afterwards every item's document holds its output (d->cur, d->len).  par:
+// the round's parity (documents are read from arena[par]; it also picks the
+// descriptor staging copy filled this round).
+int run_round(Engine *E, std::vector<RoundItem> &items, int par, FinArena &fin, double *algo_bytes)
+{
+    hipStream_t s = E->stream;
+    const int gt = kGroupTiles;
+    const size_t A = items.size();
+    // ---- classify, build descriptors, groups and the pattern pool ----------
+    std::vector<int> cls(A), lt(A);
+    size_t pool = 0;
+    int64_t cls_groups[kClasses + 1] = {0};
+    int64_t cls_docs[kClasses + 1] = {0};
+    for (size_t a = 0; a < A; ++a) {
+        const DocState &d = *items[a].d;
+        int t = 0;
+        const int mode = pattern_mode(items[a].pat, items[a].L, &t);
+        cls[a] = class_of(mode, t);
+        lt[a] = t;
+        pool += align_up((size_t)items[a].L + 16, 16) + align_up((size_t)items[a].r + 32, 16);
+        if (cls[a] < kClasses) {
+            const int64_t tiles = (d.len + kTile - 1) / kTile;
+            cls_groups[cls[a]] += (tiles + gt - 1) / gt;
+            cls_docs[cls[a]] += 1;
+        }
+    }
+    int64_t g_off[kClasses + 1], d_off[kClasses + 1];
+    int64_t G = 0, D = 0;
+    for (int c = 0; c < kClasses; ++c) {
+        g_off[c] = G;
+        d_off[c] = D;
+        G += cls_groups[c];
+        D += cls_docs[c];
+    }
+    CK(host_reserve(E->h_docs[par], A * sizeof(BatchDoc) + 64));
+    CK(host_reserve(E->h_groups[par], (size_t)G * sizeof(BatchGroup) + 64));
+    CK(host_reserve(E->h_cdocs[par], (size_t)D * sizeof(int32_t) + 64));
+    CK(host_reserve(E->h_pool[par], pool + 64));
+    CK(host_reserve(E->h_kept, A * sizeof(int64_t) + 64));
+    BatchDoc *hd = reinterpret_cast<BatchDoc *>(E->h_docs[par].p);
+    BatchGroup *hg = reinterpret_cast<BatchGroup *>(E->h_groups[par].p);
+    int32_t *hc = reinterpret_cast<int32_t *>(E->h_cdocs[par].p);
+    uint8_t *hpool = E->h_pool[par].p;
+    int64_t g_fill[kClasses], d_fill[kClasses];
+    for (int c = 0; c < kClasses; ++c) {
+        g_fill[c] = g_off[c];
+        d_fill[c] = d_off[c];
+    }
+    size_t pool_fill = 0;
+    for (size_t a = 0; a < A; ++a) {
+        const DocState &d = *items[a].d;
+        const int L = items[a].L, r = items[a].r;
+        BatchDoc bd{};
+        bd.src = d.cur;
+        bd.dst = nullptr;
+        bd.n = d.len;
+        bd.L = L;
+        bd.r = r;
+        pattern_head(items[a].pat, L, &bd.pat4, &bd.mask4);
+        bd.pat_off = (int32_t)pool_fill;
+        if (L > 0)
+            memcpy(hpool + pool_fill, items[a].pat, (size_t)L);
+        pool_fill += align_up((size_t)L + 16, 16);
+        bd.rep_off = (int32_t)pool_fill;
+        if (r > 0)
+            memcpy(hpool + pool_fill, items[a].rep, (size_t)r);
+        pool_fill += align_up((size_t)r + 32, 16);
+        const int c = cls[a];
+        if (c < kClasses) {
+            const int64_t tiles = (d.len + kTile - 1) / kTile;
+            const int64_t ng = (tiles + gt - 1) / gt;
+            bd.group0 = g_fill[c] - g_off[c];
+            bd.ngroups = (int32_t)ng;
+            for (int64_t g = 0; g < ng; ++g)
+                hg[g_fill[c] + g] = BatchGroup{(int32_t)a, (int32_t)g};
+            g_fill[c] += ng;
+            hc[d_fill[c]++] = (int32_t)a;
+        }
+        hd[a] = bd;
+    }
+    CK(dev_reserve(E->d_docs, A * sizeof(BatchDoc) + 64, false, 0, s));
+    CK(dev_reserve(E->d_groups, (size_t)G * sizeof(BatchGroup) + 64, false, 0, s));
+    CK(dev_reserve(E->d_cdocs, (size_t)D * sizeof(int32_t) + 64, false, 0, s));
+    CK(dev_reserve(E->d_pool, pool + 64, false, 0, s));
+    CK(dev_reserve(E->d_gsum, (size_t)G * kMaxSummaryWords * sizeof(uint32_t) + 64, false, 0, s));
+    CK(dev_reserve(E->d_pre, (size_t)G * sizeof(ChunkState) + 64, false, 0, s));
+    CK(dev_reserve(E->d_kept, A * sizeof(int64_t) + 64, false, 0, s));
+    if (A > 0)
+        CK(hipMemcpyAsync(E->d_docs.p, hd, A * sizeof(BatchDoc), hipMemcpyHostToDevice, s));
+    if (G > 0)
+        CK(hipMemcpyAsync(E->d_groups.p, hg, (size_t)G * sizeof(BatchGroup), hipMemcpyHostToDevice, s));
+    if (D > 0)
+        CK(hipMemcpyAsync(E->d_cdocs.p, hc, (size_t)D * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (pool_fill > 0)
+        CK(hipMemcpyAsync(E->d_pool.p, hpool, pool_fill, hipMemcpyHostToDevice, s));
+
+    auto args_for = [&](int c) {
+        BatchArgs ba{};
+        ba.docs = reinterpret_cast<const BatchDoc *>(E->d_docs.p);
+        ba.groups = reinterpret_cast<const BatchGroup *>(E->d_groups.p) + g_off[c];
+        ba.class_docs = reinterpret_cast<const int32_t *>(E->d_cdocs.p) + d_off[c];
+        ba.pool = E->d_pool.p;
+        ba.gsum = reinterpret_cast<uint32_t *>(E->d_gsum.p) + g_off[c] * kMaxSummaryWords;
+        ba.pre = reinterpret_cast<ChunkState *>(E->d_pre.p) + g_off[c];
+        ba.doc_kept = reinterpret_cast<int64_t *>(E->d_kept.p);
+        return ba;
+    };
+    static const int kClassMode[kClasses] = {kModeBorderFree, kModeTable, kModeTable, kModeTable, kModeTable};
+    static const int kClassLt[kClasses] = {1, 2, 4, 8, 16};
+    for (int c = 0; c < kClasses; ++c) {
+        if (cls_docs[c] == 0)
+            continue;
+        const BatchArgs ba = args_for(c);
+        CK(launch_batch_count(ba, kClassMode[c], kClassLt[c], gt, cls_groups[c], s));
+        CK(launch_batch_spine(ba, kClassMode[c], kClassLt[c], cls_docs[c], s));
+    }
+    if (A > 0)
+        CK(hipMemcpyAsync(E->h_kept.p, E->d_kept.p, A * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+
+    // ---- new lengths, destinations -----------------------------------------
+    const int nxt = par ^ 1;
+    const int64_t *kept = reinterpret_cast<const int64_t *>(E->h_kept.p);
+    std::vector<int64_t> new_len(A, 0);
+    size_t arena_need = 0;
+    for (size_t a = 0; a < A; ++a) {
+        if (cls[a] >= kClasses)
+            continue;
+        const DocState &d = *items[a].d;
+        new_len[a] = d.len + kept[a] * ((int64_t)items[a].r - items[a].L);
+        if (!items[a].last)
+            arena_need += align_up((size_t)new_len[a], kDocAlign);
+    }
+    CK(dev_reserve(E->arena[nxt], arena_need + kDocAlign, false, 0, s));
+    // reserve the final arena for every class document finishing this round
+    // before handing out addresses, so no address below moves before its emit
+    size_t fin_need = 0;
+    for (size_t a = 0; a < A; ++a)
+        if (cls[a] < kClasses && items[a].last)
+            fin_need += align_up((size_t)new_len[a], kDocAlign);
+    {
+        const int rc = fin.reserve(fin_need);
+        if (rc != 0)
+            return rc;
+    }
+    size_t arena_used = 0;
+    for (size_t a = 0; a < A; ++a) {
+        if (cls[a] >= kClasses)
+            continue;
+        DocState &d = *items[a].d;
+        uint8_t *dst;
+        if (items[a].last) {
+            int rc = fin.slot(new_len[a], &dst, &d.fin_off);
+            if (rc != 0)
+                return rc;
+        } else {
+            dst = E->arena[nxt].p + arena_used;
+            arena_used += align_up((size_t)new_len[a], kDocAlign);
+        }
+        hd[a].dst = dst;
+    }
+    if (A > 0)
+        CK(hipMemcpyAsync(E->d_docs.p, hd, A * sizeof(BatchDoc), hipMemcpyHostToDevice, s));
+    for (int c = 0; c < kClasses; ++c) {
+        if (cls_docs[c] == 0)
+            continue;
+        CK(launch_batch_emit(args_for(c), kClassMode[c], kClassLt[c], gt, cls_groups[c], s));
+    }
+    // ---- empty-pattern and bitmap-path documents, one at a time ------------
+    // (after a sync: the loop edits descriptors the upload above reads)
+    if (std::find_if(cls.begin(), cls.end(), [](int c) { return c >= kClasses; }) != cls.end())
+        CK(hipStreamSynchronize(s));
+    for (size_t a = 0; a < A; ++a) {
+        if (cls[a] < kClasses)
+            continue;
+        DocState &d = *items[a].d;
+        const bool last = items[a].last;
+        auto dst_for = [&](int64_t len, uint8_t **dst) -> int {
+            if (last)
+                return fin.slot(len, dst, &d.fin_off);
+            const size_t need = arena_used + align_up((size_t)len, kDocAlign) + kDocAlign;
+            if (need > E->arena[nxt].cap) {
+                // arena[nxt] already holds this round's outputs: move them (queued
+                // after their emits) and re-point their documents
+                DevBuf old = E->arena[nxt];
+                E->arena[nxt] = DevBuf{};
+                hipError_t e = dev_reserve(E->arena[nxt], need, false, 0, s);
+                if (e != hipSuccess) {
+                    E->arena[nxt] = old;
+                    return (int)e;
+                }
+                if (arena_used > 0) {
+                    e = hipMemcpyAsync(E->arena[nxt].p, old.p, arena_used, hipMemcpyDeviceToDevice, s);
+                    if (e != hipSuccess)
+                        return (int)e;
+                }
+                for (size_t b = 0; b < A; ++b)
+                    if (hd[b].dst >= old.p && hd[b].dst < old.p + old.cap)
+                        hd[b].dst = E->arena[nxt].p + (hd[b].dst - old.p);
+                e = hipStreamSynchronize(s);
+                (void)hipFree(old.p);
+                if (e != hipSuccess)
+                    return (int)e;
+            }
+            *dst = E->arena[nxt].p + arena_used;
+            arena_used += align_up((size_t)len, kDocAlign);
+            return 0;
+        };
+        int64_t len = 0;
+        int rc = run_other(E, d, items[a].pat, items[a].L, items[a].rep, items[a].r, &len, dst_for);
+        if (rc != 0)
+            return rc;
+        new_len[a] = len;
+        // record where the output went (dst_for may have moved other documents)
+        hd[a].dst = last ? E->fin.p + d.fin_off : E->arena[nxt].p + (arena_used - align_up((size_t)len, kDocAlign));
+    }
+    for (size_t a = 0; a < A; ++a) {
+        DocState &d = *items[a].d;
+        *algo_bytes += (double)d.len + (double)new_len[a];
+        d.len = new_len[a];
+        d.cur = items[a].last ? E->fin.p + d.fin_off : hd[a].dst;
+        items[a].new_len = new_len[a];
+    }
+    return 0;
+}
+
 int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t *doc_lens, const int *pair_counts,
                const uint8_t *const *pats, const int *pat_lens, const uint8_t *const *reps, const int *rep_lens,
                const uint8_t **out_base, size_t *out_offsets, size_t *out_lens, double *stats)
@@ -357,25 +625,14 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
     for (int i = 0; i < num_docs; ++i)
         st[(size_t)i].cur = E->arena[0].p + pieces[(size_t)i].dst_off;
 
-    size_t fin_used = 0;
-    auto fin_slot = [&](int64_t len, uint8_t **dst, size_t *off) -> int {
-        const size_t need = fin_used + align_up((size_t)len, kDocAlign) + kDocAlign;
-        if (need > E->fin.cap) {
-            // live documents in the arenas keep their addresses; only fin moves
-            CK(dev_reserve(E->fin, need, true, fin_used, s));
-        }
-        *off = fin_used;
-        *dst = E->fin.p + fin_used;
-        fin_used += align_up((size_t)len, kDocAlign);
-        return 0;
-    };
+    FinArena fin{E};
     // documents without pairs are final as uploaded
     for (int i = 0; i < num_docs; ++i) {
         DocState &d = st[(size_t)i];
         if (d.npairs > 0)
             continue;
         uint8_t *dst = nullptr;
-        int rc = fin_slot(d.len, &dst, &d.fin_off);
+        int rc = fin.slot(d.len, &dst, &d.fin_off);
         if (rc != 0)
             return rc;
         if (d.len > 0)
@@ -383,238 +640,27 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
     }
 
     CK(hipEventRecord(E->ev0, s));
-    const int gt = kGroupTiles;
     double algo_bytes = 0;
-    std::vector<int> active;
+    std::vector<RoundItem> items;
     for (int j = 0; j < max_pairs; ++j) {
-        active.clear();
-        for (int i = 0; i < num_docs; ++i)
-            if (st[(size_t)i].npairs > j)
-                active.push_back(i);
-        const size_t A = active.size();
-        // ---- classify, build descriptors, groups and the pattern pool ----------
-        std::vector<int> cls(A), lt(A);
-        size_t pool = 0;
-        int64_t cls_groups[kClasses + 1] = {0};
-        int64_t cls_docs[kClasses + 1] = {0};
-        for (size_t a = 0; a < A; ++a) {
-            const DocState &d = st[(size_t)active[a]];
+        items.clear();
+        for (int i = 0; i < num_docs; ++i) {
+            DocState &d = st[(size_t)i];
+            if (d.npairs <= j)
+                continue;
             const int64_t q = d.pair0 + j;
-            int t = 0;
-            const int mode = pattern_mode(pats[q], pat_lens[q], &t);
-            cls[a] = class_of(mode, t);
-            lt[a] = t;
-            pool += align_up((size_t)pat_lens[q] + 16, 16) + align_up((size_t)rep_lens[q] + 32, 16);
-            if (cls[a] < kClasses) {
-                const int64_t tiles = (d.len + kTile - 1) / kTile;
-                cls_groups[cls[a]] += (tiles + gt - 1) / gt;
-                cls_docs[cls[a]] += 1;
-            }
+            items.push_back(RoundItem{&d, pats[q], pat_lens[q], reps[q], rep_lens[q], d.npairs == j + 1, 0});
         }
-        int64_t g_off[kClasses + 1], d_off[kClasses + 1];
-        int64_t G = 0, D = 0;
-        for (int c = 0; c < kClasses; ++c) {
-            g_off[c] = G;
-            d_off[c] = D;
-            G += cls_groups[c];
-            D += cls_docs[c];
-        }
-        const int par = j & 1;
-        CK(host_reserve(E->h_docs[par], A * sizeof(BatchDoc) + 64));
-        CK(host_reserve(E->h_groups[par], (size_t)G * sizeof(BatchGroup) + 64));
-        CK(host_reserve(E->h_cdocs[par], (size_t)D * sizeof(int32_t) + 64));
-        CK(host_reserve(E->h_pool[par], pool + 64));
-        CK(host_reserve(E->h_kept, A * sizeof(int64_t) + 64));
-        BatchDoc *hd = reinterpret_cast<BatchDoc *>(E->h_docs[par].p);
-        BatchGroup *hg = reinterpret_cast<BatchGroup *>(E->h_groups[par].p);
-        int32_t *hc = reinterpret_cast<int32_t *>(E->h_cdocs[par].p);
-        uint8_t *hpool = E->h_pool[par].p;
-        int64_t g_fill[kClasses], d_fill[kClasses];
-        for (int c = 0; c < kClasses; ++c) {
-            g_fill[c] = g_off[c];
-            d_fill[c] = d_off[c];
-        }
-        size_t pool_fill = 0;
-        for (size_t a = 0; a < A; ++a) {
-            const DocState &d = st[(size_t)active[a]];
-            const int64_t q = d.pair0 + j;
-            const int L = pat_lens[q], r = rep_lens[q];
-            BatchDoc bd{};
-            bd.src = d.cur;
-            bd.dst = nullptr;
-            bd.n = d.len;
-            bd.L = L;
-            bd.r = r;
-            pattern_head(pats[q], L, &bd.pat4, &bd.mask4);
-            bd.pat_off = (int32_t)pool_fill;
-            if (L > 0)
-                memcpy(hpool + pool_fill, pats[q], (size_t)L);
-            pool_fill += align_up((size_t)L + 16, 16);
-            bd.rep_off = (int32_t)pool_fill;
-            if (r > 0)
-                memcpy(hpool + pool_fill, reps[q], (size_t)r);
-            pool_fill += align_up((size_t)r + 32, 16);
-            const int c = cls[a];
-            if (c < kClasses) {
-                const int64_t tiles = (d.len + kTile - 1) / kTile;
-                const int64_t ng = (tiles + gt - 1) / gt;
-                bd.group0 = g_fill[c] - g_off[c];
-                bd.ngroups = (int32_t)ng;
-                for (int64_t g = 0; g < ng; ++g)
-                    hg[g_fill[c] + g] = BatchGroup{(int32_t)a, (int32_t)g};
-                g_fill[c] += ng;
-                hc[d_fill[c]++] = (int32_t)a;
-            }
-            hd[a] = bd;
-        }
-        CK(dev_reserve(E->d_docs, A * sizeof(BatchDoc) + 64, false, 0, s));
-        CK(dev_reserve(E->d_groups, (size_t)G * sizeof(BatchGroup) + 64, false, 0, s));
-        CK(dev_reserve(E->d_cdocs, (size_t)D * sizeof(int32_t) + 64, false, 0, s));
-        CK(dev_reserve(E->d_pool, pool + 64, false, 0, s));
-        CK(dev_reserve(E->d_gsum, (size_t)G * kMaxSummaryWords * sizeof(uint32_t) + 64, false, 0, s));
-        CK(dev_reserve(E->d_pre, (size_t)G * sizeof(ChunkState) + 64, false, 0, s));
-        CK(dev_reserve(E->d_kept, A * sizeof(int64_t) + 64, false, 0, s));
-        if (A > 0)
-            CK(hipMemcpyAsync(E->d_docs.p, hd, A * sizeof(BatchDoc), hipMemcpyHostToDevice, s));
-        if (G > 0)
-            CK(hipMemcpyAsync(E->d_groups.p, hg, (size_t)G * sizeof(BatchGroup), hipMemcpyHostToDevice, s));
-        if (D > 0)
-            CK(hipMemcpyAsync(E->d_cdocs.p, hc, (size_t)D * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        if (pool_fill > 0)
-            CK(hipMemcpyAsync(E->d_pool.p, hpool, pool_fill, hipMemcpyHostToDevice, s));
-
-        auto args_for = [&](int c) {
-            BatchArgs ba{};
-            ba.docs = reinterpret_cast<const BatchDoc *>(E->d_docs.p);
-            ba.groups = reinterpret_cast<const BatchGroup *>(E->d_groups.p) + g_off[c];
-            ba.class_docs = reinterpret_cast<const int32_t *>(E->d_cdocs.p) + d_off[c];
-            ba.pool = E->d_pool.p;
-            ba.gsum = reinterpret_cast<uint32_t *>(E->d_gsum.p) + g_off[c] * kMaxSummaryWords;
-            ba.pre = reinterpret_cast<ChunkState *>(E->d_pre.p) + g_off[c];
-            ba.doc_kept = reinterpret_cast<int64_t *>(E->d_kept.p);
-            return ba;
-        };
-        static const int kClassMode[kClasses] = {kModeBorderFree, kModeTable, kModeTable, kModeTable, kModeTable};
-        static const int kClassLt[kClasses] = {1, 2, 4, 8, 16};
-        for (int c = 0; c < kClasses; ++c) {
-            if (cls_docs[c] == 0)
-                continue;
-            const BatchArgs ba = args_for(c);
-            CK(launch_batch_count(ba, kClassMode[c], kClassLt[c], gt, cls_groups[c], s));
-            CK(launch_batch_spine(ba, kClassMode[c], kClassLt[c], cls_docs[c], s));
-        }
-        if (A > 0)
-            CK(hipMemcpyAsync(E->h_kept.p, E->d_kept.p, A * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        CK(hipStreamSynchronize(s));
-
-        // ---- new lengths, destinations -----------------------------------------
-        const int nxt = (j + 1) & 1;
-        const int64_t *kept = reinterpret_cast<const int64_t *>(E->h_kept.p);
-        std::vector<int64_t> new_len(A, 0);
-        size_t arena_need = 0;
-        for (size_t a = 0; a < A; ++a) {
-            if (cls[a] >= kClasses)
-                continue;
-            const DocState &d = st[(size_t)active[a]];
-            const int64_t q = d.pair0 + j;
-            new_len[a] = d.len + kept[a] * ((int64_t)rep_lens[q] - pat_lens[q]);
-            if (d.npairs != j + 1)
-                arena_need += align_up((size_t)new_len[a], kDocAlign);
-        }
-        CK(dev_reserve(E->arena[nxt], arena_need + kDocAlign, false, 0, s));
-        // reserve the final arena for every class document finishing this round
-        // before handing out addresses, so no address below moves before its emit
-        size_t fin_need = 0;
-        for (size_t a = 0; a < A; ++a)
-            if (cls[a] < kClasses && st[(size_t)active[a]].npairs == j + 1)
-                fin_need += align_up((size_t)new_len[a], kDocAlign);
-        if (fin_used + fin_need + kDocAlign > E->fin.cap)
-            CK(dev_reserve(E->fin, fin_used + fin_need + kDocAlign, true, fin_used, s));
-        size_t arena_used = 0;
-        for (size_t a = 0; a < A; ++a) {
-            if (cls[a] >= kClasses)
-                continue;
-            DocState &d = st[(size_t)active[a]];
-            uint8_t *dst;
-            if (d.npairs == j + 1) {
-                int rc = fin_slot(new_len[a], &dst, &d.fin_off);
-                if (rc != 0)
-                    return rc;
-            } else {
-                dst = E->arena[nxt].p + arena_used;
-                arena_used += align_up((size_t)new_len[a], kDocAlign);
-            }
-            hd[a].dst = dst;
-        }
-        if (A > 0)
-            CK(hipMemcpyAsync(E->d_docs.p, hd, A * sizeof(BatchDoc), hipMemcpyHostToDevice, s));
-        for (int c = 0; c < kClasses; ++c) {
-            if (cls_docs[c] == 0)
-                continue;
-            CK(launch_batch_emit(args_for(c), kClassMode[c], kClassLt[c], gt, cls_groups[c], s));
-        }
-        // ---- empty-pattern and bitmap-path documents, one at a time ------------
-        // (after a sync: the loop edits descriptors the upload above reads)
-        if (std::find_if(cls.begin(), cls.end(), [](int c) { return c >= kClasses; }) != cls.end())
-            CK(hipStreamSynchronize(s));
-        for (size_t a = 0; a < A; ++a) {
-            if (cls[a] < kClasses)
-                continue;
-            DocState &d = st[(size_t)active[a]];
-            const int64_t q = d.pair0 + j;
-            const bool last = d.npairs == j + 1;
-            auto dst_for = [&](int64_t len, uint8_t **dst) -> int {
-                if (last)
-                    return fin_slot(len, dst, &d.fin_off);
-                const size_t need = arena_used + align_up((size_t)len, kDocAlign) + kDocAlign;
-                if (need > E->arena[nxt].cap) {
-                    // arena[nxt] already holds this round's outputs: move them (queued
-                    // after their emits) and re-point their documents
-                    DevBuf old = E->arena[nxt];
-                    E->arena[nxt] = DevBuf{};
-                    hipError_t e = dev_reserve(E->arena[nxt], need, false, 0, s);
-                    if (e != hipSuccess) {
-                        E->arena[nxt] = old;
-                        return (int)e;
-                    }
-                    if (arena_used > 0) {
-                        e = hipMemcpyAsync(E->arena[nxt].p, old.p, arena_used, hipMemcpyDeviceToDevice, s);
-                        if (e != hipSuccess)
-                            return (int)e;
-                    }
-                    for (size_t b = 0; b < A; ++b)
-                        if (hd[b].dst >= old.p && hd[b].dst < old.p + old.cap)
-                            hd[b].dst = E->arena[nxt].p + (hd[b].dst - old.p);
-                    e = hipStreamSynchronize(s);
-                    (void)hipFree(old.p);
-                    if (e != hipSuccess)
-                        return (int)e;
-                }
-                *dst = E->arena[nxt].p + arena_used;
-                arena_used += align_up((size_t)len, kDocAlign);
-                return 0;
-            };
-            int64_t len = 0;
-            int rc = run_other(E, d, pats[q], pat_lens[q], reps[q], rep_lens[q], &len, dst_for);
-            if (rc != 0)
-                return rc;
-            new_len[a] = len;
-            // record where the output went (dst_for may have moved other documents)
-            hd[a].dst = last ? E->fin.p + d.fin_off : E->arena[nxt].p + (arena_used - align_up((size_t)len, kDocAlign));
-        }
-        for (size_t a = 0; a < A; ++a) {
-            DocState &d = st[(size_t)active[a]];
-            algo_bytes += (double)d.len + (double)new_len[a];
-            d.len = new_len[a];
-            d.cur = d.npairs == j + 1 ? E->fin.p + d.fin_off : hd[a].dst;
-        }
+        const int rc = run_round(E, items, j & 1, fin, &algo_bytes);
+        if (rc != 0)
+            return rc;
     }
     CK(hipEventRecord(E->ev1, s));
 
     // ---- results ----------------------------------------------------------------
-    CK(host_reserve(E->h_out, fin_used + 64));
-    if (fin_used > 0)
-        CK(hipMemcpyAsync(E->h_out.p, E->fin.p, fin_used, hipMemcpyDeviceToHost, s));
+    CK(host_reserve(E->h_out, fin.used + 64));
+    if (fin.used > 0)
+        CK(hipMemcpyAsync(E->h_out.p, E->fin.p, fin.used, hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
     for (int i = 0; i < num_docs; ++i) {
         out_offsets[i] = st[(size_t)i].fin_off;
@@ -630,6 +676,128 @@ int engine_run(Engine *E, int num_docs, const uint8_t *const *docs, const size_t
         stats[3] = max_pairs;
     }
     return 0;
+}
+
+// ---- live runs (continuous batching) ----------------------------------------
+struct LiveDoc {
+    DocState d;
+    LiveReq req;
+    int next = 0;   // the pair this round applies
+};
+
+int engine_live(Engine *E, const std::function<bool(std::vector<LiveReq> &, bool)> &admit,
+                const std::function<void(void *, int, const uint8_t *, size_t)> &finish)
+{
+    hipStream_t s = E->stream;
+    std::vector<LiveDoc> live;
+    std::vector<LiveReq> incoming;
+    std::vector<RoundItem> items;
+    std::vector<Piece> pieces;
+    int cur = 0;           // arena holding the live documents
+    size_t cur_used = 0;   // their bytes, packed from its start
+    auto fail_all = [&](int rc) {
+        for (LiveDoc &ld : live)
+            finish(ld.req.tag, rc, nullptr, 0);
+        live.clear();
+        cur_used = 0;
+        return rc;
+    };
+    for (;;) {
+        incoming.clear();
+        if (!admit(incoming, live.empty()) && live.empty())
+            return 0;
+        // ---- newcomers: uploaded behind the live documents --------------------
+        // (every request with pairs joins the live set, an empty document too)
+        pieces.clear();
+        size_t add = 0;
+        bool joins = false;
+        for (const LiveReq &r : incoming) {
+            if (r.npairs == 0) {
+                finish(r.tag, 0, r.doc, r.len);
+                continue;
+            }
+            joins = true;
+            pieces.push_back(Piece{add, r.doc, r.len});
+            add += align_up(r.len, kDocAlign);
+        }
+        if (joins) {
+            uint8_t *const old = E->arena[cur].p;
+            hipError_t e = dev_reserve(E->arena[cur], cur_used + add + kDocAlign, true, cur_used, s);
+            if (e == hipSuccess)
+                e = host_reserve(E->h_in, add + 1);
+            if (e != hipSuccess) {
+                for (const LiveReq &r : incoming)
+                    if (r.npairs > 0)
+                        finish(r.tag, (int)e, nullptr, 0);
+                if (live.empty())
+                    continue;
+                return fail_all((int)e);
+            }
+            if (E->arena[cur].p != old)
+                for (LiveDoc &ld : live)
+                    ld.d.cur = E->arena[cur].p + (ld.d.cur - old);
+            if (add > 0) {
+                e = stage_and_upload(E->h_in.p, E->arena[cur].p + cur_used, pieces, add, s);
+                if (e != hipSuccess) {
+                    for (const LiveReq &r : incoming)
+                        if (r.npairs > 0)
+                            finish(r.tag, (int)e, nullptr, 0);
+                    return fail_all((int)e);
+                }
+            }
+            size_t k = 0;
+            for (const LiveReq &r : incoming) {
+                if (r.npairs == 0)
+                    continue;
+                LiveDoc ld;
+                ld.d.cur = E->arena[cur].p + cur_used + pieces[k++].dst_off;
+                ld.d.len = (int64_t)r.len;
+                ld.d.npairs = r.npairs;
+                ld.req = r;
+                live.push_back(ld);
+            }
+            cur_used += add;
+        }
+        if (live.empty())
+            continue;
+        // ---- one round: every live document applies its next pair --------------
+        items.clear();
+        for (LiveDoc &ld : live) {
+            const int q = ld.next;
+            items.push_back(RoundItem{&ld.d, ld.req.pats[q], ld.req.pat_lens[q], ld.req.reps[q], ld.req.rep_lens[q],
+                                      q + 1 == ld.req.npairs, 0});
+        }
+        FinArena fin{E};
+        double algo = 0;
+        int rc = run_round(E, items, cur, fin, &algo);
+        if (rc != 0)
+            return fail_all(rc);
+        // ---- finished documents go back; the others continue from arena[cur ^ 1]
+        hipError_t e = host_reserve(E->h_out, fin.used + 64);
+        if (e == hipSuccess && fin.used > 0)
+            e = hipMemcpyAsync(E->h_out.p, E->fin.p, fin.used, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        if (e != hipSuccess)
+            return fail_all((int)e);
+        const int nxt = cur ^ 1;
+        size_t used = 0;
+        size_t w = 0;
+        for (size_t i = 0; i < live.size(); ++i) {
+            LiveDoc &ld = live[i];
+            if (ld.next + 1 == ld.req.npairs) {
+                finish(ld.req.tag, 0, E->h_out.p + ld.d.fin_off, (size_t)ld.d.len);
+                continue;
+            }
+            ld.next += 1;
+            const size_t end = (size_t)(ld.d.cur - E->arena[nxt].p) + align_up((size_t)ld.d.len, kDocAlign);
+            used = std::max(used, end);
+            live[w++] = ld;
+        }
+        live.resize(w);
+        cur = nxt;
+        cur_used = used;
+    }
 }
 
 // ---- device-driven runs -------------------------------------------------------
@@ -1253,3 +1421,18 @@ CR_API void cr_batch_destroy(void *b)
 }
 
 } // extern "C"
+
+namespace cr {
+
+int batch_live(void *eng, const std::function<bool(std::vector<LiveReq> &, bool)> &admit,
+               const std::function<void(void *, int, const uint8_t *, size_t)> &finish)
+{
+    if (!eng)
+        return -1;
+    Engine *E = static_cast<Engine *>(eng);
+    std::lock_guard<std::mutex> lock(E->mtx);
+    DeviceGuard dg(E->device);
+    return engine_live(E, admit, finish);
+}
+
+} // namespace cr

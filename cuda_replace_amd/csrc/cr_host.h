@@ -4,7 +4,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <stdint.h>
+
 #include <algorithm>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -39,4 +42,29 @@ inline void parallel_memcpy(void *dst, const void *src, size_t n)
         t.join();
 }
 
+// A request of a live run: one document and its ordered pair list (caller-owned
+// buffers, valid until its finish callback has returned).
+struct LiveReq {
+    const uint8_t *doc;
+    size_t len;
+    int npairs;
+    const uint8_t *const *pats;
+    const int *pat_lens;
+    const uint8_t *const *reps;
+    const int *rep_lens;
+    void *tag;
+};
+
+// Continuous batching on one batch engine (cr_batch_create): a live set of documents,
+// each at its own pair, advances one host-driven round at a time.  Before every
+// round admit(out, idle) appends the requests that join (idle: nothing is live; it
+// may block then, and returning false while idle ends the run); they are uploaded
+// behind the live documents and take their first pair in that round.  As a document
+// applies its last pair, finish(tag, rc, out, out_len) gets its result (out valid
+// during the call).  Returns 0, or the error of a round (every live document is
+// finished with it first).
+int batch_live(void *eng, const std::function<bool(std::vector<LiveReq> &, bool)> &admit,
+               const std::function<void(void *, int, const uint8_t *, size_t)> &finish);
+
 } // namespace cr
+

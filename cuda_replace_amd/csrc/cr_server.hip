@@ -2,14 +2,17 @@
 //
 // Requests (one document and its ordered pair list) enter a FIFO.  Each worker
 // thread owns a batch engine (its own HIP stream, resident arenas and pinned
-// staging); four workers by default, so one batch's host packing and copies
-// overlap other batches' transfers and kernels.  When a worker is free it admits queued requests in arrival order while
-// the server's HBM budget and the batch size cap allow, runs them as one batch, and
-// completes each request with its own copy of the result.  Requests that arrive
-// while batches are on the GPU wait only until a worker frees up, and with two or
-// more workers two batches overlap on the device (one uploading or downloading while
-// the other computes).  A request larger than the whole budget runs alone once
-// nothing else holds budget.
+// staging); four workers by default.  A worker runs a live set of documents, each
+// at its own pair, one round at a time: before every round it admits queued
+// requests in arrival order while the server's HBM budget and the worker's live-set
+// cap allow, uploads them behind the live documents and starts them on their first
+// pair in that same round; a document that has applied its last pair returns at
+// once with its own copy of the result.  So an arrival waits at most for the round
+// in flight, not for a whole batch, and the workers' rounds overlap on the device
+// (one uploading or downloading while another computes).  A request larger than the
+// whole budget runs alone once nothing else holds budget.  CR_SERVER_LIVE=0 runs
+// the earlier form instead: a free worker takes the queued requests as one batch and
+// returns them together.
 //
 // No counterpart in the reference (a library without a serving layer).
 #include "../../include/cuda_replace.h"
@@ -51,6 +54,7 @@ struct Ticket {
 
 struct Server {
     int device = 0;
+    bool live = true;
     size_t budget = 0;
     size_t max_batch = 0;
     std::mutex mtx;
@@ -155,6 +159,76 @@ void worker_loop(Server *S, int w)
     }
 }
 
+// Admission test for a queued request: it fits the budget and the worker's live-set
+// cap, or it may run alone (larger than either, with nothing else holding budget).
+bool admissible(const Server *S, const Ticket *t, size_t held, bool worker_idle, bool *alone)
+{
+    const bool fits = S->in_use + t->reserve <= S->budget && (worker_idle || held + t->reserve <= S->max_batch);
+    *alone = !fits && worker_idle && S->in_use == 0;
+    return fits || *alone;
+}
+
+void live_worker_loop(Server *S, int w)
+{
+    (void)hipSetDevice(S->device);
+    void *eng = S->engines[(size_t)w];
+    size_t held = 0;    // budget held by this worker's live requests
+    size_t nlive = 0;
+    bool solo = false;  // an oversized request runs alone
+    auto admit = [&](std::vector<cr::LiveReq> &out, bool idle) -> bool {
+        std::unique_lock<std::mutex> lk(S->mtx);
+        for (;;) {
+            while (!S->queue.empty() && !solo) {
+                Ticket *t = S->queue.front();
+                bool alone = false;
+                if (!admissible(S, t, held, nlive == 0 && out.empty(), &alone))
+                    break;
+                S->queue.pop_front();
+                S->in_use += t->reserve;
+                held += t->reserve;
+                ++nlive;
+                solo = alone;
+                t->t_start = Clock::now();
+                t->batch_docs = (int)nlive;
+                out.push_back(cr::LiveReq{t->doc, t->len, (int)t->pats.size(), t->pats.data(), t->pat_lens.data(),
+                                          t->reps.data(), t->rep_lens.data(), t});
+            }
+            if (!out.empty() || !idle)
+                return true;
+            if (S->stop && S->queue.empty())
+                return false;
+            S->cv.wait(lk);
+        }
+    };
+    auto finish = [&](void *tag, int rc, const uint8_t *res, size_t len) {
+        Ticket *t = static_cast<Ticket *>(tag);
+        int trc = rc;
+        if (rc == 0 && len > 0) {
+            t->out = (uint8_t *)malloc(len);
+            if (t->out) {
+                cr::parallel_memcpy(t->out, res, len);
+                t->out_len = len;
+            } else {
+                trc = -2;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(S->mtx);
+            S->in_use -= t->reserve;
+            held -= t->reserve;
+            --nlive;
+            if (nlive == 0)
+                solo = false;
+        }
+        S->cv.notify_all();
+        complete(t, trc);
+    };
+    for (;;) {
+        if (cr::batch_live(eng, admit, finish) == 0)
+            return;   // stopped with nothing queued or live
+    }
+}
+
 void shutdown(Server *S)
 {
     {
@@ -199,8 +273,14 @@ CR_API int cr_server_create(void **ps, size_t hbm_budget_bytes, int workers, siz
         }
         S->engines.push_back(eng);
     }
-    for (int w = 0; w < nw; ++w)
-        S->workers.emplace_back(worker_loop, S, w);
+    const char *lv = getenv("CR_SERVER_LIVE");
+    S->live = !(lv && atoi(lv) == 0);
+    for (int w = 0; w < nw; ++w) {
+        if (S->live)
+            S->workers.emplace_back(live_worker_loop, S, w);
+        else
+            S->workers.emplace_back(worker_loop, S, w);
+    }
     *ps = S;
     return 0;
 }
@@ -244,8 +324,9 @@ CR_API int cr_server_submit(void *srv, const uint8_t *doc, size_t len, int npair
 }
 
 // Blocks until the request is done.  *out (malloc'd, owned by the ticket) and
-// *out_len receive the result; times (optional, 3 doubles): ms from submit to the
-// start of its batch, ms from submit to completion, documents in its batch.
+// *out_len receive the result; times (optional, 3 doubles): ms from submit to its
+// admission, ms from submit to completion, and the documents live on its worker
+// when it was admitted, itself included (the batch size in the batch-at-a-time form).
 CR_API int cr_server_wait(void *ticket, const uint8_t **out, size_t *out_len, double *times)
 {
     if (!ticket)

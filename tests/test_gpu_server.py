@@ -61,3 +61,31 @@ def test_server_tiny_budget_runs_requests_alone(lib):
         assert all(t.times["batch_docs"] == 1 for t in ts)   # every request exceeds half the budget
     for (doc, prs), got in zip(reqs, outs):
         assert got == ro.apply_batch(doc, prs)
+
+
+def test_arrivals_join_a_running_worker(lib):
+    """One worker: a long request (64 pairs over 8 MiB) is in its rounds when short
+    requests arrive; they are admitted into the next round, so they come back
+    before the long one, each equal to the oracle."""
+    import time
+    text = workloads.zipf_text(8 << 20, seed=5)
+    pool = workloads.pair_pool(text[: 1 << 20], 64, seed=13)
+    long_pairs = [pool[i % len(pool)] for i in range(64)]
+    rng = random.Random(3)
+    shorts = []
+    for _ in range(12):
+        at = rng.randrange(0, len(text) - 50000)
+        shorts.append((text[at:at + rng.randrange(100, 50000)], [rng.choice(pool) for _ in range(rng.randint(1, 3))]))
+    with Server(lib, workers=1) as srv:
+        t0 = time.perf_counter()
+        lt = srv.submit(text, long_pairs)
+        time.sleep(0.005)
+        sts = [(time.perf_counter(), srv.submit(d, p)) for d, p in shorts]
+        outs = [(at, t, t.result()) for at, t in sts]
+        lres = lt.result()
+        long_done = t0 + lt.times["latency_ms"] / 1e3
+        for (d, p), (at, t, got) in zip(shorts, outs):
+            assert got == ro.apply_batch(d, p)
+            assert at + t.times["latency_ms"] / 1e3 < long_done, (t.times, lt.times)
+            assert t.times["batch_docs"] >= 2, t.times   # joined the live long request
+    assert lres == ro.apply_batch(text, long_pairs)
