@@ -36,6 +36,12 @@ serving  = open-loop Poisson arrivals (1000 requests/s for --serve-seconds) of
            section; the same arrivals through the batch-at-a-time form beside it;
            the server's capacity on a burst of the same mix and the per-call loop
            (one cuda_replace_unified per pair) on the same mix, and their ratio.
+bitmap   = bitmap-mode passes over the 1 GiB inputs: a*17 -> X over all-'a' (a
+           self-overlapping pattern longer than the carry tables take) and a 300-byte
+           slice of the text -> 6 bytes, beside the 2-state table pass on all-'a';
+           HIP-event times and the ratio to the table pass, outputs checked.
+small_passes = device passes over one 64 KiB document for each carry-table class
+           and a border-free pattern (the fixed cost of a small pass).
 streaming = the text repeated to --stream-gib GiB, streamed through the pipelined
            chunked path in 256 MiB chunks (fragment -> 12 bytes), host to host,
            checked against bytes.replace of the 1 GiB piece.
@@ -380,6 +386,78 @@ def c1_bench(lib, h, prep: dict, dev, runs: int = 20) -> dict:
     return res
 
 
+def _device_pass(lib, h, d_src, n, d_dst, pat, rep):
+    """One cr_replace_device pass; (output length, HIP-event ms of its launches)."""
+    out_len = C.c_size_t()
+    kms = C.c_float()
+    rc = lib.lib.cr_replace_device(h, C.c_void_p(d_src.data_ptr()), n, C.c_void_p(d_dst.data_ptr()), d_dst.numel(),
+                                   C.cast(C.c_char_p(pat), C.POINTER(C.c_uint8)), len(pat),
+                                   C.cast(C.c_char_p(rep), C.POINTER(C.c_uint8)), len(rep), -len(pat),
+                                   C.byref(out_len), None, C.byref(kms))
+    if rc != 0:
+        raise RuntimeError(f"cr_replace_device rc={rc}")
+    return out_len.value, kms.value
+
+
+def bitmap_bench(lib, h, text: bytes, all_a_dev, text_dev, dev, runs: int = 5) -> dict:
+    """Bitmap-mode passes over the 1 GiB inputs (patterns the carry tables and the
+    256-byte halo do not take) beside the 2-state carry-table pass on the same
+    all-'a' input; each output checked byte for byte against bytes.replace."""
+    import torch
+    n = text_dev.numel()
+    long_pat = text[5000:5300]
+    cases = [("all_a_a17_X", all_a_dev, b"a" * n, b"a" * 17, b"X"),
+             ("text_300B_pattern", text_dev, text, long_pat, b"<LONG>"),
+             ("all_a_aa_X_table_path", all_a_dev, b"a" * n, b"aa", b"X")]
+    res = {"workload": "1 GiB all-'a' with a*17 -> X (self-overlapping, 17 bytes); 1 GiB text with a 300-byte "
+                       "slice of itself -> 6 B; the 2-state table pass aa -> X beside them",
+           "timing": "HIP events around the pass's launches on the library stream, median of %d" % runs}
+    for name, d_src, src, pat, rep in cases:
+        d_dst = torch.empty(n + 64, dtype=torch.uint8, device=dev)
+        _device_pass(lib, h, d_src, n, d_dst, pat, rep)
+        ts = [_device_pass(lib, h, d_src, n, d_dst, pat, rep)[1] for _ in range(runs)]
+        ol, _ = _device_pass(lib, h, d_src, n, d_dst, pat, rep)
+        exact = d_dst[:ol].cpu().numpy().tobytes() == src.replace(pat, rep)
+        med = statistics.median(ts)
+        res[name] = {"pattern_mode": lib.pattern_mode(pat), "kernel_ms_median": round(med, 4),
+                     "algo_GBps": round((n + ol) / med / 1e6, 1),
+                     "hbm_frac": round((n + ol) / (med / 1e3) / HBM_PEAK, 4), "bit_exact": bool(exact)}
+        log(f"[bench] bitmap {name}: {med:.3f} ms, bit_exact={exact}")
+        del d_dst
+    base = res["all_a_aa_X_table_path"]["kernel_ms_median"]
+    for k in ("all_a_a17_X", "text_300B_pattern"):
+        res[k]["vs_table_path"] = round(res[k]["kernel_ms_median"] / base, 2)
+    res["bit_exact"] = all(res[c[0]]["bit_exact"] for c in cases)
+    return res
+
+
+def small_pass_bench(lib, h, dev, runs: int = 20) -> dict:
+    """Device passes over one 64 KiB document of biased {a, b} for each carry-table
+    class (2, 4, 8 and 16 states) and a border-free pattern: the fixed cost of a
+    small pass (count, single-launch spine, emit)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(1)
+    doc = np.where(rng.random(64 << 10) < 0.9, 97, 98).astype(np.uint8).tobytes()
+    d_src = torch.frombuffer(bytearray(doc), dtype=torch.uint8).to(dev)
+    d_dst = torch.empty(len(doc) * 11 + 64, dtype=torch.uint8, device=dev)
+    res = {"workload": "64 KiB of biased {a,b} (90% a), one pass per pattern",
+           "timing": "HIP events around the pass's launches, median of %d" % runs}
+    exact = True
+    for pat, rep in [(b"aa", b"X"), (b"aaa", b"Q"), (b"a" * 7, b"Y"), (b"a" * 12, b"Z"), (b"ab", b"c")]:
+        for _ in range(3):
+            _device_pass(lib, h, d_src, len(doc), d_dst, pat, rep)
+        ts = [_device_pass(lib, h, d_src, len(doc), d_dst, pat, rep)[1] for _ in range(runs)]
+        ol, _ = _device_pass(lib, h, d_src, len(doc), d_dst, pat, rep)
+        exact &= d_dst[:ol].cpu().numpy().tobytes() == doc.replace(pat, rep)
+        res[f"L{len(pat)}_{'table' if lib.pattern_mode(pat) == 2 else 'border_free'}"] = {
+            "kernel_us_median": round(statistics.median(ts) * 1e3, 1)}
+    res["bit_exact"] = bool(exact)
+    log(f"[bench] small passes: " + ", ".join(f"{k} {v['kernel_us_median']} us" for k, v in res.items()
+                                                if isinstance(v, dict)))
+    return res
+
+
 # ------------------------------------------------------------------ main
 def batch_bench(lib, ro, num_docs: int) -> dict:
     """Batched uneven documents vs the per-call loop (see module docstring)."""
@@ -635,7 +713,13 @@ def main() -> int:
     ap.add_argument("--no-session-batch", action="store_true", help="skip the 128-pair session batch")
     ap.add_argument("--session-gib", type=float, default=2.0)
     ap.add_argument("--no-c1", action="store_true", help="skip the 64 MiB configurations")
+    ap.add_argument("--no-bitmap", action="store_true", help="skip the bitmap-mode passes")
+    ap.add_argument("--no-small", action="store_true", help="skip the 64 KiB passes")
+    ap.add_argument("--only-device-passes", action="store_true",
+                    help="only the device-resident passes (main, 64 MiB, bitmap, small): for profiling runs")
     args = ap.parse_args()
+    if args.only_device_passes:
+        args.no_cpu = args.no_batch = args.no_serve = args.no_stream = args.no_session_batch = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the replace path runs on a single MI355X; "
               "sharding over several GPUs is not built", file=sys.stderr)
@@ -753,6 +837,8 @@ def main() -> int:
         exact &= ok
         log(f"[bench] {name}: out_len={ol} bit_exact={ok}")
     c1 = None if c1_prep is None else c1_bench(lib, h, c1_prep, dev)
+    bitmap = None if args.no_bitmap else bitmap_bench(lib, h, text, d_src["all_a"], d_src["text"], dev)
+    small = None if args.no_small else small_pass_bench(lib, h, dev)
     lib.lib.cuda_replace_close(h)
     batched = None if args.no_batch else batch_bench(lib, ro, args.batch_docs)
     serving = None if args.no_serve else serve_bench(lib, ro, text, args.serve_seconds, workers=args.serve_workers)
@@ -804,8 +890,9 @@ def main() -> int:
         "bit_exact": exact,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": traffic,
-                     "kernel": "one border-free replace pass = count_kernel + spine_kernel + emit_kernel "
-                               "(text expand and shrink), timed by HIP events around the three launches"},
+                     "kernel": "one border-free replace pass (text expand and shrink) = bf_sweep_kernel, one "
+                               "persistent launch (count phases ahead of emit phases, a scanner workgroup "
+                               "publishing group prefixes), timed by HIP events around its launch"},
         "cpu_baseline": {"value": round(cpu_in / cpu_t / 1e9, 3), "unit": "GB/s", "cores": 1,
                          "kind": "port",
                          "sample": f"CPython bytes.replace, single thread, one pass over each of the "
@@ -825,6 +912,8 @@ def main() -> int:
                          "python": platform.python_version()},
         "device_copy_GBps": round(copy_gbs, 1),
         "c1": c1,
+        "bitmap": bitmap,
+        "small_passes": small,
         "batched": batched,
         "serving": serving,
         "streaming": streaming,
@@ -833,7 +922,8 @@ def main() -> int:
         "detail": detail,
     }
     print(json.dumps(line), flush=True)
-    ok = exact and all(x is None or x["bit_exact"] for x in (c1, batched, serving, streaming, session_batch))
+    ok = exact and all(x is None or x["bit_exact"] for x in (c1, bitmap, small, batched, serving, streaming,
+                                                               session_batch))
     return 0 if ok else 1
 
 
