@@ -773,11 +773,16 @@ int engine_live(Engine *E, const std::function<bool(std::vector<LiveReq> &, bool
         if (rc != 0)
             return fail_all(rc);
         // ---- finished documents go back; the others continue from arena[cur ^ 1]
-        hipError_t e = host_reserve(E->h_out, fin.used + 64);
-        if (e == hipSuccess && fin.used > 0)
-            e = hipMemcpyAsync(E->h_out.p, E->fin.p, fin.used, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(s);
+        // (a round without finished documents needs no wait here: the next round's
+        // readback orders everything after it)
+        hipError_t e = hipSuccess;
+        if (fin.used > 0) {
+            e = host_reserve(E->h_out, fin.used + 64);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(E->h_out.p, E->fin.p, fin.used, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(s);
+        }
         if (e != hipSuccess)
             return fail_all((int)e);
         const int nxt = cur ^ 1;

@@ -885,15 +885,17 @@ __global__ void __launch_bounds__(kWave) spine_top_kernel(ReplaceParams p, int64
     }
 }
 
-// Carry-table spine of a pass with at most kSpineThreads groups, one launch of
-// 64 * ceil(groups / 64) threads: lane = group.  Each wave runs one ordered
-// inclusive scan of its groups' tables, with only as many steps as it has groups;
-// wave 0 then scans the waves' totals (as many steps as there are waves) and hands
-// each wave its entering state; a lane's exclusive state is its left neighbour's
-// inclusive table evaluated at that state.  A small document's pass (a few groups)
-// costs a few table compositions instead of a full 1024-group walk.
+// Carry-table spine of a pass of up to kSpineThreads * v groups in one launch of
+// 64 * ceil(groups / (64 v)) threads: lane l takes the v consecutive groups from
+// l * v and composes their tables.  Each wave runs one ordered inclusive scan of its
+// lanes' tables, with only as many steps as it has lanes in use; wave 0 then scans
+// the waves' totals the same way and hands each wave its entering state; a lane's
+// entering state is its left neighbour's inclusive table at that state, and its
+// groups' states follow by walking their tables again.  A small document's pass (a
+// few groups) costs a few compositions, and a pass of thousands of groups one
+// launch instead of a reduce, a block scan and a spine launch.
 template <int LT>
-__global__ void __launch_bounds__(kSpineThreads) spine_small_kernel(ReplaceParams p, int64_t groups)
+__global__ void __launch_bounds__(kSpineThreads) spine_small_kernel(ReplaceParams p, int64_t groups, int v)
 {
     __shared__ uint64_t s_wt[kSpineWaves][LT];
     __shared__ int64_t s_k[kSpineWaves];
@@ -902,16 +904,23 @@ __global__ void __launch_bounds__(kSpineThreads) spine_small_kernel(ReplaceParam
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
     const int nwaves = (int)(blockDim.x / kWave);
-    const int64_t gi = tid;
-    Table<LT> t = tidentity<LT>();
-    if (gi < groups) {
+    const int64_t g0 = (int64_t)tid * v;   // this lane's first group
+    Table64<LT> mine = tidentity64<LT>();
+    for (int i = 0; i < v; ++i) {
+        const int64_t gi = g0 + i;
+        if (gi >= groups)
+            break;
+        Table<LT> t;
 #pragma unroll
         for (int c = 0; c < LT; ++c)
             t.e[c] = p.gsum[gi * LT + c];
+        mine = tcompose64<LT>(mine, widen<LT>(t));
     }
-    const int64_t left = groups - (int64_t)wave * kWave;
-    const int act = left < kWave ? (int)left : kWave;   // >= 1: the launch covers `groups`
-    Table64<LT> inc = widen<LT>(t);
+    const int64_t per_wave = (int64_t)kWave * v;
+    const int64_t left = groups - (int64_t)wave * per_wave;
+    const int64_t lanes_used = (left + v - 1) / v;
+    const int act = lanes_used < kWave ? (int)lanes_used : kWave;   // >= 1: the launch covers `groups`
+    Table64<LT> inc = mine;
     for (int d = 1; d < act; d <<= 1) {
         Table64<LT> nb;
 #pragma unroll
@@ -966,7 +975,6 @@ __global__ void __launch_bounds__(kSpineThreads) spine_small_kernel(ReplaceParam
         }
     }
     __syncthreads();
-    const int64_t kw = s_k[wave];
     const uint32_t cw = s_c[wave];
     uint64_t e = (uint64_t)cw << kCarryShift;   // lane 0: nothing of this wave before it
 #pragma unroll
@@ -975,12 +983,24 @@ __global__ void __launch_bounds__(kSpineThreads) spine_small_kernel(ReplaceParam
         if (lane > 0 && (uint32_t)c == cw)
             e = u;
     }
-    if (gi < groups) {
+    int64_t k = s_k[wave] + (int64_t)(e & kCountMask);
+    uint32_t carry = (uint32_t)(e >> kCarryShift);
+    for (int i = 0; i < v; ++i) {
+        const int64_t gi = g0 + i;
+        if (gi >= groups)
+            break;
         ChunkState cs;
-        cs.k_base = kw + (int64_t)(e & kCountMask);
-        cs.carry = (int)(e >> kCarryShift);
+        cs.k_base = k;
+        cs.carry = (int)carry;
         cs.pad = 0;
         p.chunks_pre[gi] = cs;
+        Table<LT> t;
+#pragma unroll
+        for (int c = 0; c < LT; ++c)
+            t.e[c] = p.gsum[gi * LT + c];
+        const uint32_t en = tsel<LT>(t, carry);
+        k += (int64_t)(en & 0xffffffu);
+        carry = en >> 24;
     }
 }
 
@@ -3686,9 +3706,14 @@ hipError_t spine_blocks_t(const ReplaceParams &p, int64_t groups, hipStream_t s)
     constexpr int64_t kPass = spine_block_groups<MODE, LT>();
     const int64_t nb = (groups + kPass - 1) / kPass;
     if constexpr (MODE == kModeTable) {
-        if (groups <= kSpineThreads) {
-            const unsigned threads = (unsigned)((groups + kWave - 1) / kWave * kWave);
-            hipLaunchKernelGGL((spine_small_kernel<LT>), dim3(1), dim3(threads), 0, s, p, groups);
+        // one launch while every lane composes at most kSmallSpineV groups (fewer for
+        // 8- and 16-state tables, whose compositions cost O(states^2))
+        constexpr int kSmallSpineV = LT <= 4 ? 16 : 4;
+        if (groups <= (int64_t)kSpineThreads * kSmallSpineV) {
+            const int v = (int)((groups + kSpineThreads - 1) / kSpineThreads);
+            const int64_t lanes = (groups + v - 1) / v;
+            const unsigned threads = (unsigned)((lanes + kWave - 1) / kWave * kWave);
+            hipLaunchKernelGGL((spine_small_kernel<LT>), dim3(1), dim3(threads), 0, s, p, groups, v);
             return hipGetLastError();
         }
     }

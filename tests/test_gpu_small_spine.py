@@ -1,8 +1,9 @@
 """Carry-table passes of at most 1024 groups (gpu): the single-launch small spine.
 
 A self-overlapping pattern of up to 16 bytes runs with carry tables of 2, 4, 8 or 16
-states; a pass of at most 1024 groups resolves the groups' states in one launch
-that scans the group tables wave by wave (cuda_replace.cu:905-968 is the serial
+states; a pass of at most 1024 groups (16384 for 2- and 4-state tables, 4096 for
+8 and 16 states, with several groups per lane) resolves the groups' states in one
+launch that scans the group tables wave by wave (cuda_replace.cu:905-968 is the serial
 greedy these tables replace).  Inputs are biased {a, b} so long runs of 'a' make
 the carry between groups matter; sizes put 1, 2, 63, 64, 65, 128, 1000 and 1024
 groups in a pass, with 1025 groups (the blocked spine) beside them.  Every output,
@@ -38,6 +39,20 @@ def test_small_spine_every_group_count(lib, pat, rep):
         n = g * GROUP - (0 if g % 2 else 777)
         src = biased(n, g * 31 + len(pat))
         say("small spine", pat[:8], len(pat), g, n)
+        assert lib.unified(src, pat, rep) == src.replace(pat, rep), (pat, g)
+
+
+@pytest.mark.parametrize("pat,rep", [(b"aa", b"X"), (b"abab", b"Q"), (b"a" * 12, b"")], ids=["L2", "L4", "L12"])
+def test_small_spine_several_groups_per_lane(lib, pat, rep):
+    """Passes of 1025 to 16384 groups: every lane of the single spine launch composes
+    several consecutive groups (up to 16 for 2- and 4-state tables, 4 for 8 and 16
+    states; the blocked spine beyond)."""
+    for g in (1025, 2047, 2048, 4095, 4096, 4097, 16383, 16385):
+        if len(pat) > 4 and g > 4097:
+            continue
+        n = g * GROUP - (3 if g % 2 else 0)
+        src = biased(n, g + len(pat))
+        say("small spine v>1", len(pat), g, n)
         assert lib.unified(src, pat, rep) == src.replace(pat, rep), (pat, g)
 
 
