@@ -696,6 +696,35 @@ def stream_bench(lib, text: bytes, frag: bytes, rep: bytes, piece: bytes, gib: i
     }
 
 
+BF_FORMS = {
+    "single_launch": "bf_sweep_kernel, one persistent launch (count phases ahead of emit phases, a scanner "
+                     "workgroup publishing group prefixes)",
+    "three_launch_64B": "bf_count_kernel + spine + bf_emit_kernel, three launches at 64 source bytes per thread",
+    "tiles1_32B": "count_kernel + spine + emit_kernel at 32 source bytes per thread, 1 tile per group",
+    "tiles2_32B": "count_kernel + spine + emit_kernel at 32 source bytes per thread, 2 tiles per group",
+    "tiles4_32B": "count_kernel + spine + emit_kernel at 32 source bytes per thread, 4 tiles per group",
+}
+
+
+def size_bucket(n):
+    """The autotuner's size bucket names (input bytes of one pass)."""
+    mib = 1 << 20
+    for lim, name in ((4 * mib, "<4MiB"), (32 * mib, "4-32MiB"), (256 * mib, "32-256MiB"),
+                      (2048 * mib, "256MiB-2GiB")):
+        if n < lim:
+            return name
+    return ">=2GiB"
+
+
+def bf_pass_label(lib, n):
+    """Names the border-free pass form the autotuner chose for inputs of n bytes."""
+    form = next((e.get("form") for e in lib.autotune_report()
+                 if e["class"] == "border_free" and e["size"] == size_bucket(n)), None)
+    what = BF_FORMS.get(form, "the default form (tuning off or not yet run)")
+    return (f"one border-free replace pass (text expand and shrink), the autotuner's form "
+            f"{form}: {what}; timed by HIP events around the pass's launches")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -890,9 +919,7 @@ def main() -> int:
         "bit_exact": exact,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": traffic,
-                     "kernel": "one border-free replace pass (text expand and shrink) = bf_sweep_kernel, one "
-                               "persistent launch (count phases ahead of emit phases, a scanner workgroup "
-                               "publishing group prefixes), timed by HIP events around its launch"},
+                     "kernel": bf_pass_label(lib, n)},
         "cpu_baseline": {"value": round(cpu_in / cpu_t / 1e9, 3), "unit": "GB/s", "cores": 1,
                          "kind": "port",
                          "sample": f"CPython bytes.replace, single thread, one pass over each of the "

@@ -46,9 +46,20 @@ constexpr size_t kAllocGranule = 2u << 20;   // device buffers grow in 2 MiB ste
 // CR_AUTOTUNE=0 keeps the default.
 constexpr int kTuneCands[3] = {1, 2, 4};
 
+// Border-free passes have more forms than a group size; the tuner measures them all:
+// the single persistent launch (64 source bytes per thread, 16 KiB groups, counts
+// two rounds ahead of the emit), the same kernels as three launches (the emit
+// recomputes candidates and takes its carry from a left halo), and the 32-byte-per-
+// thread count and emit with 1, 2 or 4 tiles per group (the count hands every
+// segment's candidate mask to the emit).
+enum BfVariant { kBfSweep = 0, kBfLean = 1, kBfTiles1 = 2, kBfTiles2 = 3, kBfTiles4 = 4, kBfVariants = 5 };
+constexpr int kBfVariantTiles[kBfVariants] = {kGroupTiles, kGroupTiles, 1, 2, 4};
+
 struct TuneEntry {
     int gt = 0;
-    float ms[3] = {0, 0, 0};   // per candidate, best of two
+    float ms[3] = {0, 0, 0};   // per candidate, best of three
+    int variant = -1;          // border-free class: the chosen BfVariant
+    float vms[kBfVariants] = {0, 0, 0, 0, 0};
 };
 
 struct Tuner {
@@ -373,6 +384,31 @@ int sweep_spin_limit()
     return v < 1 ? 1 : v;
 }
 
+// The form of a border-free pass (BfVariant); *tune is set when this pass should
+// measure every form.  CR_BF_VARIANT=0..4 fixes it; CR_SWEEP=0 keeps the single
+// launch out; untuned, passes of CR_SWEEP_MIN bytes (64 MiB) or more take the single
+// launch and smaller ones the three launches of the same kernels.
+int choose_bf_variant(int device, int64_t n, bool *tune)
+{
+    *tune = false;
+    const int64_t smin = sweep_min_bytes();
+    const bool sweep_ok = smin != INT64_MAX;
+    if (const char *e = getenv("CR_BF_VARIANT")) {
+        const int v = atoi(e);
+        if (v >= 0 && v < kBfVariants)
+            return v == kBfSweep && !sweep_ok ? kBfLean : v;
+    }
+    const int dflt = n >= smin ? kBfSweep : kBfLean;
+    const int b = size_bucket(n);
+    Tuner &t = tuner();
+    std::lock_guard<std::mutex> lk(t.mtx);
+    auto it = t.best.find(std::make_tuple(device, 0, b));
+    if (it != t.best.end() && it->second.variant >= 0)
+        return it->second.variant == kBfSweep && !sweep_ok ? kBfLean : it->second.variant;
+    *tune = b > 0 && autotune_enabled();
+    return dflt;
+}
+
 // The sweep's words, cleared when (re)allocated.
 hipError_t ensure_sweep(Session *s, int64_t groups)
 {
@@ -552,8 +588,12 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     CK(upload_pattern(s, pat, L, rep, r));
     const int64_t num_tiles = (n + kTile - 1) / kTile;
     bool tune = false;
-    // border-free patterns run the lean pass, whose group is fixed at 16 KiB
-    const int gt = mode == kModeBorderFree ? kGroupTiles : choose_group_tiles(s->device, mode, table_size, n, &tune);
+    // border-free patterns: the form the tuner chose (its group size follows);
+    // other patterns: the tuned group size
+    const int bf_var = mode == kModeBorderFree ? choose_bf_variant(s->device, n, &tune) : -1;
+    const bool bf_lean = bf_var == kBfSweep || bf_var == kBfLean;   // 64 bytes per thread
+    const int gt = mode == kModeBorderFree ? kBfVariantTiles[bf_var]
+                                           : choose_group_tiles(s->device, mode, table_size, n, &tune);
     const int64_t num_groups = groups_of(num_tiles, gt);
     const int64_t cg = chunk_groups();
     const int64_t num_chunks = (num_groups + cg - 1) / cg;
@@ -609,7 +649,9 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     p.chunks_pre = s->d_pre;
     p.info = s->d_info;
     p.bitmap = s->d_bitmap;
-    p.masks = mode == kModeTable && s->bitmap_words >= (size_t)num_tiles * (kTile / 32) ? s->d_bitmap : nullptr;
+    // the 32-byte-per-thread count hands its candidate masks to the emit
+    const bool have_mask_buf = s->bitmap_words >= (size_t)num_tiles * (kTile / 32);
+    p.masks = (mode == kModeTable || (mode == kModeBorderFree && !bf_lean)) && have_mask_buf ? s->d_bitmap : nullptr;
 
     if (mode == kModeBitmap) {
         // candidates -> serial greedy (gives the exact count) -> emit
@@ -662,8 +704,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             for (int64_t c = 0; c < num_chunks; ++c) {
                 const int64_t g0 = c * cg;
                 const int64_t gn = num_groups - g0 < cg ? num_groups - g0 : cg;
-                hipError_t e = mode == kModeBorderFree ? launch_bf_count(p, g0, gn, s->stream)
-                                                       : launch_count(p, mode, table_size, gt, g0, gn, s->stream);
+                hipError_t e = bf_lean ? launch_bf_count(p, g0, gn, s->stream)
+                                       : launch_count(p, mode, table_size, gt, g0, gn, s->stream);
                 if (e != hipSuccess)
                     return e;
                 e = num_chunks == 1 ? launch_spine_blocks(p, mode, table_size, gn, s->stream)
@@ -671,8 +713,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                 if (e != hipSuccess)
                     return e;
                 if (emit) {
-                    e = mode == kModeBorderFree ? launch_bf_emit(p, g0, gn, s->stream)
-                                                : launch_emit(p, mode, table_size, gt, g0, gn, s->stream);
+                    e = bf_lean ? launch_bf_emit(p, g0, gn, s->stream)
+                                : launch_emit(p, mode, table_size, gt, g0, gn, s->stream);
                     if (e != hipSuccess)
                         return e;
                 }
@@ -715,8 +757,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         p.dst = dst;
         // border-free passes of a single chunk: one persistent launch when large enough
         bool swept = false;
-        if (!counted && mode == kModeBorderFree && num_chunks == 1 && n >= sweep_min_bytes() &&
-            ensure_sweep(s, num_groups) == hipSuccess) {
+        if (!counted && bf_var == kBfSweep && num_chunks == 1 && ensure_sweep(s, num_groups) == hipSuccess) {
             if (kernel_ms)
                 CK(hipEventRecord(s->ev0, s->stream));
             CK(launch_bf_sweep(p, s->d_sweep, sweep_ahead(), sweep_spin_limit(), s->stream));
@@ -739,8 +780,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         }
         if (!swept) {
             if (counted) {
-                CK(mode == kModeBorderFree ? launch_bf_emit(p, 0, num_groups, s->stream)
-                                           : launch_emit(p, mode, table_size, gt, 0, num_groups, s->stream));
+                CK(bf_lean ? launch_bf_emit(p, 0, num_groups, s->stream)
+                           : launch_emit(p, mode, table_size, gt, 0, num_groups, s->stream));
             } else {
                 if (kernel_ms)
                     CK(hipEventRecord(s->ev0, s->stream));
@@ -761,7 +802,67 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         }
         if (kernel_ms)
             CK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
-        if (tune && num_chunks == 1) {
+        if (tune && num_chunks == 1 && mode == kModeBorderFree) {
+            // every form recomputes the same summaries, states, records and output
+            // (a single launch that gives up leaves what it wrote equal to the
+            // rest), so the pass's results above stay valid.  Three interleaved
+            // sweeps, best time per form; another form replaces the default only
+            // when it is more than 3% faster.
+            TuneEntry te;
+            te.gt = kGroupTiles;
+            const ReplaceParams p0 = p;
+            constexpr float kGaveUp = 1e30f;
+            for (int sweep = 0; sweep < 3; ++sweep) {
+                for (int v = 0; v < kBfVariants; ++v) {
+                    ReplaceParams q = p0;
+                    const int vg = kBfVariantTiles[v];
+                    const int64_t ng = groups_of(num_tiles, vg);
+                    q.num_groups = ng;
+                    q.masks = v >= kBfTiles1 && have_mask_buf ? s->d_bitmap : nullptr;
+                    float ms = kGaveUp;
+                    if (v == kBfSweep) {
+                        if (sweep_min_bytes() == INT64_MAX || ensure_sweep(s, ng) != hipSuccess) {
+                            (void)hipGetLastError();
+                            te.vms[v] = kGaveUp;
+                            continue;
+                        }
+                        CK(next_epoch(s));
+                        q.epoch = s->epoch;
+                        CK(hipEventRecord(s->ev0, s->stream));
+                        CK(launch_bf_sweep(q, s->d_sweep, sweep_ahead(), sweep_spin_limit(), s->stream));
+                        CK(hipEventRecord(s->ev1, s->stream));
+                        CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * ng, 8, hipMemcpyDeviceToHost,
+                                          s->stream));
+                        CK(hipStreamSynchronize(s->stream));
+                        const uint64_t fv = *(const uint64_t *)(s->h_misc + 96);
+                        if (fv != (((uint64_t)s->epoch << kEpochShift) | 1u))
+                            CK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                    } else {
+                        const bool lean = v == kBfLean;
+                        CK(hipEventRecord(s->ev0, s->stream));
+                        CK(lean ? launch_bf_count(q, 0, ng, s->stream)
+                                : launch_count(q, mode, table_size, vg, 0, ng, s->stream));
+                        CK(launch_spine_blocks(q, mode, table_size, ng, s->stream));
+                        CK(lean ? launch_bf_emit(q, 0, ng, s->stream)
+                                : launch_emit(q, mode, table_size, vg, 0, ng, s->stream));
+                        CK(hipEventRecord(s->ev1, s->stream));
+                        CK(hipEventSynchronize(s->ev1));
+                        CK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                    }
+                    te.vms[v] = sweep == 0 || ms < te.vms[v] ? ms : te.vms[v];
+                }
+            }
+            const int dflt = n >= sweep_min_bytes() ? kBfSweep : kBfLean;
+            te.variant = dflt;
+            float best = te.vms[dflt];
+            for (int v = 0; v < kBfVariants; ++v) {
+                if (te.vms[v] < best && te.vms[v] < 0.97f * te.vms[dflt]) {
+                    te.variant = v;
+                    best = te.vms[v];
+                }
+            }
+            record_tuning(s->device, mode, table_size, n, te);
+        } else if (tune && num_chunks == 1) {
             // every candidate recomputes the same summaries, states, records and
             // output, so the pass's results above stay valid.  Three interleaved
             // sweeps (so clock and cache drift hit every candidate alike), best
@@ -1774,7 +1875,10 @@ int cr_autotune_set(const uint8_t *pat, int pat_len, size_t n, int group_tiles)
         return -1;
     int lt = 0;
     const int c = tune_class(pattern_mode(pat, pat_len, &lt), lt);
-    if (c < 0 || !(group_tiles == 0 || group_tiles == 1 || group_tiles == 2 || group_tiles == 4))
+    // border-free patterns: 1, 2 or 4 pin the 32-byte-per-thread form with that group
+    // size, 8 the three-launch 64-byte form, 16 the single launch
+    const bool bf_codes = c == 0 && (group_tiles == 8 || group_tiles == 16);
+    if (c < 0 || !(group_tiles == 0 || group_tiles == 1 || group_tiles == 2 || group_tiles == 4 || bf_codes))
         return -1;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -1783,10 +1887,19 @@ int cr_autotune_set(const uint8_t *pat, int pat_len, size_t n, int group_tiles)
     Tuner &t = tuner();
     std::lock_guard<std::mutex> lk(t.mtx);
     const auto key = std::make_tuple(dev, c, size_bucket((int64_t)n));
-    if (group_tiles == 0)
+    if (group_tiles == 0) {
         t.best.erase(key);
-    else
-        t.best[key] = TuneEntry{group_tiles, {0, 0, 0}};
+    } else if (c == 0) {
+        TuneEntry te;
+        te.gt = kGroupTiles;
+        te.variant = group_tiles == 16 ? kBfSweep : group_tiles == 8 ? kBfLean
+                   : group_tiles == 1 ? kBfTiles1 : group_tiles == 2 ? kBfTiles2 : kBfTiles4;
+        t.best[key] = te;
+    } else {
+        TuneEntry te;
+        te.gt = group_tiles;
+        t.best[key] = te;
+    }
     return 0;
 }
 
@@ -1806,14 +1919,27 @@ int cr_autotune_report(char *buf, size_t cap)
         Tuner &t = tuner();
         std::lock_guard<std::mutex> lk(t.mtx);
         bool first = true;
+        static const char *kVariant[kBfVariants] = {"single_launch", "three_launch_64B", "tiles1_32B",
+                                                    "tiles2_32B", "tiles4_32B"};
         for (const auto &kv : t.best) {
-            char line[256];
-            snprintf(line, sizeof line,
-                     "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"group_tiles\": %d, "
-                     "\"ms\": {\"1\": %.4f, \"2\": %.4f, \"4\": %.4f}}",
-                     first ? "" : ", ", std::get<0>(kv.first), kClassName[std::get<1>(kv.first)],
-                     kBucket[std::get<2>(kv.first)], kv.second.gt, kv.second.ms[0], kv.second.ms[1],
-                     kv.second.ms[2]);
+            char line[512];
+            const TuneEntry &te = kv.second;
+            if (te.variant >= 0) {
+                auto f = [](float v) { return v >= 1e29f ? -1.0 : (double)v; };   // -1: gave up
+                snprintf(line, sizeof line,
+                         "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"form\": \"%s\", "
+                         "\"ms\": {\"single_launch\": %.4f, \"three_launch_64B\": %.4f, \"tiles1_32B\": %.4f, "
+                         "\"tiles2_32B\": %.4f, \"tiles4_32B\": %.4f}}",
+                         first ? "" : ", ", std::get<0>(kv.first), kClassName[std::get<1>(kv.first)],
+                         kBucket[std::get<2>(kv.first)], kVariant[te.variant], f(te.vms[0]), f(te.vms[1]),
+                         f(te.vms[2]), f(te.vms[3]), f(te.vms[4]));
+            } else {
+                snprintf(line, sizeof line,
+                         "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"group_tiles\": %d, "
+                         "\"ms\": {\"1\": %.4f, \"2\": %.4f, \"4\": %.4f}}",
+                         first ? "" : ", ", std::get<0>(kv.first), kClassName[std::get<1>(kv.first)],
+                         kBucket[std::get<2>(kv.first)], te.gt, te.ms[0], te.ms[1], te.ms[2]);
+            }
             out += line;
             first = false;
         }

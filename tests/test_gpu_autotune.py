@@ -1,10 +1,10 @@
-"""Group-size autotuner on the GPU: every group size (1, 2 or 4 tiles of 8 KiB per
-workgroup) must give the oracle's output, around the tile and group boundaries of
-each size, for carry tables of 2/4/8/16 states (border-free patterns run the lean
-pass, whose group is fixed at 16 KiB, whatever size is pinned); and the first pass
-in a tuned size bucket must record a choice with a measured time for every
-candidate (the default size unless another is more than 3% faster), and still
-return the exact output."""
+"""Autotuner on the GPU.  Every carry-table group size (1, 2 or 4 tiles of 8 KiB per
+workgroup) and every form of a border-free pass (the 32-byte-per-thread count and
+emit with 1, 2 or 4 tiles per group, the 64-byte three-launch pass, the single
+launch) must give the oracle's output around the tile and group boundaries of each
+size, plain and seeded with prefix queries; and the first pass in a tuned size
+bucket must record a choice with a measured time for every candidate (the default
+unless another is more than 3% faster), and still return the exact output."""
 import random
 
 import pytest
@@ -29,10 +29,17 @@ def pinned(lib):
     lib.autotune_reset()
 
 
-@pytest.mark.parametrize("gt", [1, 2, 4])
+def _pinnable(lib, pat, gt):
+    """1/2/4 pin any tuned class; 8 and 16 (the 64-byte forms) only border-free ones."""
+    return gt in (1, 2, 4) or lib.pattern_mode(pat) == 1
+
+
+@pytest.mark.parametrize("gt", [1, 2, 4, 8, 16])
 def test_every_group_size_exact(lib, pinned, gt):
     rng = random.Random(60 + gt)
     for pat, rep in PATS:
+        if not _pinnable(lib, pat, gt):
+            continue
         lib.autotune_set(pat, 0, gt)           # bucket of small inputs
         for n in SIZES:
             src = _src(rng, n)
@@ -41,11 +48,13 @@ def test_every_group_size_exact(lib, pinned, gt):
         assert lib.unified(src, pat, rep) == src.replace(pat, rep), (gt, pat, "all-a")
 
 
-@pytest.mark.parametrize("gt", [1, 2, 4])
+@pytest.mark.parametrize("gt", [1, 2, 4, 8, 16])
 def test_group_size_seeded_and_query(lib, pinned, gt):
     rng = random.Random(70 + gt)
     src = _src(rng, 50_000)
     for pat, rep in PATS[:6]:
+        if not _pinnable(lib, pat, gt):
+            continue
         lib.autotune_set(pat, 0, gt)
         for seed in (-5, 0, 8190, 16383, 32770):
             with Session(lib, src) as s:
@@ -66,13 +75,24 @@ def test_first_tuned_pass_measures_every_candidate(lib, pinned):
         assert lib.unified(src, pat, rep) == want      # uses the choice
     rep_ = lib.autotune_report()
     classes = {e["class"]: e for e in rep_}
-    assert set(classes) == {"table2"}      # border-free passes are not tuned
-    for e in rep_:
-        assert e["size"] == "4-32MiB" and e["group_tiles"] in (1, 2, 4)
-        ms = e["ms"]
-        assert all(ms[k] > 0 for k in ("1", "2", "4"))
-        best = min(ms.values())
-        if e["group_tiles"] == 2:   # the default stays unless another size is >3% faster
-            assert best >= 0.97 * ms["2"] or ms["2"] == best
-        else:
-            assert ms[str(e["group_tiles"])] == best and best < 0.97 * ms["2"]
+    assert set(classes) == {"border_free", "table2"}
+    t = classes["table2"]
+    assert t["size"] == "4-32MiB" and t["group_tiles"] in (1, 2, 4)
+    ms = t["ms"]
+    assert all(ms[k] > 0 for k in ("1", "2", "4"))
+    best = min(ms.values())
+    if t["group_tiles"] == 2:   # the default stays unless another size is >3% faster
+        assert best >= 0.97 * ms["2"] or ms["2"] == best
+    else:
+        assert ms[str(t["group_tiles"])] == best and best < 0.97 * ms["2"]
+    b = classes["border_free"]
+    forms = ("single_launch", "three_launch_64B", "tiles1_32B", "tiles2_32B", "tiles4_32B")
+    assert b["size"] == "4-32MiB" and b["form"] in forms
+    ms = b["ms"]
+    assert all(ms[k] > 0 or (k == "single_launch" and ms[k] == -1) for k in forms)   # -1: gave up
+    timed = {k: v for k, v in ms.items() if v > 0}
+    dflt = "three_launch_64B"          # the default under 64 MiB
+    if b["form"] == dflt:
+        assert min(timed.values()) >= 0.97 * ms[dflt] or ms[dflt] == min(timed.values())
+    else:
+        assert ms[b["form"]] == min(timed.values()) and ms[b["form"]] < 0.97 * ms[dflt]
