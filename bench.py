@@ -699,6 +699,8 @@ def stream_bench(lib, text: bytes, frag: bytes, rep: bytes, piece: bytes, gib: i
 BF_FORMS = {
     "single_launch": "bf_sweep_kernel, one persistent launch (count phases ahead of emit phases, a scanner "
                      "workgroup publishing group prefixes)",
+    "single_launch_lds": "bf_sweep_keep_kernel, one persistent launch holding each group in LDS from its count "
+                         "to its emit one round later (a scanner workgroup publishing group prefixes)",
     "three_launch_64B": "bf_count_kernel + spine + bf_emit_kernel, three launches at 64 source bytes per thread",
     "tiles1_32B": "count_kernel + spine + emit_kernel at 32 source bytes per thread, 1 tile per group",
     "tiles2_32B": "count_kernel + spine + emit_kernel at 32 source bytes per thread, 2 tiles per group",

@@ -293,7 +293,8 @@ hipError_t launch_bf_emit(const ReplaceParams &p, int64_t first_group, int64_t g
 // sweep: sweep_words(num_groups) 64-bit words holding no word tagged with p.epoch;
 // the kept total lands in p.stats (tagged), or sweep[2 * groups] = epoch << 48 | 1
 // when a wait gave up (then nothing of the pass is valid: run the three launches).
-hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, int spin_limit, hipStream_t s);
+hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, int spin_limit, bool keep,
+                           hipStream_t s);
 int64_t sweep_words(int64_t groups);
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s);

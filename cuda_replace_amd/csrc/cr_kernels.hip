@@ -1749,11 +1749,27 @@ struct BfLds {
     int *tile;           // 4
 };
 
-// Emit of the staged group g, whose kept starts before it number k_excl: per-tile
-// records and the output.  Ends after the output stores are issued; the caller
-// synchronizes before the LDS is reused.
-template <bool FULL4, bool IMAGE>
-__device__ __forceinline__ void bf_emit_staged(const ReplaceParams &p, int64_t g, int64_t k_excl, const BfLds &s,
+// The kept starts before a group, for an emit that has them at its start (KnownK)
+// or waits for them only where the output address is first needed (a persistent
+// launch's prefix word, polled by thread 0; see LateK beside the sweep kernels).
+// poll() is called by every thread before a barrier, get() after it.
+struct KnownK {
+    int64_t k;
+    __device__ __forceinline__ void poll() {}
+    __device__ __forceinline__ bool get(int64_t &out) const
+    {
+        out = k;
+        return true;
+    }
+};
+
+// Emit of the staged group g (kept starts before it from kp): per-tile records and
+// the output.  The output image in LDS does not depend on the kept starts before
+// the group, only its address does, so a late prefix is waited for after the image
+// is built.  Ends after the output stores are issued (the caller synchronizes
+// before the LDS is reused); false if the prefix wait gave up.
+template <bool FULL4, bool IMAGE, int CAP = kBfOutCap, class KP>
+__device__ __forceinline__ bool bf_emit_staged(const ReplaceParams &p, int64_t g, KP &kp, const BfLds &s,
                                                uint4 rep_v, const BfRep &rp)
 {
     const int tid = threadIdx.x;
@@ -1826,35 +1842,77 @@ __device__ __forceinline__ void bf_emit_staged(const ReplaceParams &p, int64_t g
     }
     if (tid == 127 || tid == 255)
         s.tile[2 + (tid >> 7)] = max(blk_prev, lq);   // last kept start up to the tile's end
-    // output of the group: [out_base, out_base + out_len) in the destination
+    // output of the group: [out_base, out_base + out_len) in the destination, out_base =
+    // G0 + c0 + k_excl * diff
     const int64_t out_len = (int64_t)valid - c0 + c_next + (int64_t)ktot * diff;
-    const int64_t out_base = G0 + c0 + k_excl * diff;
     const int jend = (valid - seg) < kBfSeg ? ((valid - seg) > 0 ? valid - seg : 0) : kBfSeg;
     int j = c_t < jend ? c_t : jend;
     const int64_t o = (int64_t)(seg + j - c0) + (int64_t)kbefore * diff;   // my first output byte
     uint64_t M = m;
-    __syncthreads();   // s.tile
+    const bool image = IMAGE && p.write_output && out_len + 16 <= CAP && r <= kRepLds;
 
-    if (tid == 0 && p.info) {
-        TileInfo a;
-        a.k_excl = k_excl;
-        a.last_abs = s.tile[2] >= 0 ? G0 + s.tile[2] : -1;
-        a.carry_in = c0;
-        a.mode = kModeBorderFree;
-        p.info[t0] = a;
-        if (t0 + 1 < p.num_tiles) {
-            TileInfo b;
-            b.k_excl = k_excl + s.tile[0];
-            b.last_abs = s.tile[3] >= kTile ? G0 + s.tile[3] : -1;
-            b.carry_in = s.tile[1] < kTile ? s.tile[1] : kTile;
-            b.mode = kModeBorderFree;
-            p.info[t0 + 1] = b;
+    auto records = [&](int64_t k_excl) {
+        if (tid == 0 && p.info) {
+            TileInfo a;
+            a.k_excl = k_excl;
+            a.last_abs = s.tile[2] >= 0 ? G0 + s.tile[2] : -1;
+            a.carry_in = c0;
+            a.mode = kModeBorderFree;
+            p.info[t0] = a;
+            if (t0 + 1 < p.num_tiles) {
+                TileInfo b;
+                b.k_excl = k_excl + s.tile[0];
+                b.last_abs = s.tile[3] >= kTile ? G0 + s.tile[3] : -1;
+                b.carry_in = s.tile[1] < kTile ? s.tile[1] : kTile;
+                b.mode = kModeBorderFree;
+                p.info[t0 + 1] = b;
+            }
         }
-    }
-    if (!p.write_output)
-        return;
-
+    };
     const uint8_t *const s_src = s.in + kHalo + seg;
+    int64_t k_excl = 0;
+
+    if (image) {
+        // my pieces into the LDS image, then the workgroup stores the image
+        int oo = (int)o;
+        for (;;) {
+            const int qk = M ? (int)__builtin_ctzll(M) : jend;
+            bf_copy_run(s.out + oo, s_src + j, qk - j);
+            oo += qk - j;
+            if (qk >= jend)
+                break;
+            bf_put_rep_lds(s.out + oo, rp, rep_v, s.rep, r);
+            oo += r;
+            j = qk + L;
+            M &= M - 1;
+            if (j >= jend)
+                break;
+        }
+        kp.poll();
+        __syncthreads();   // the image and s.tile; the prefix
+        if (!kp.get(k_excl))
+            return false;
+        records(k_excl);
+        uint8_t *const out_first = p.dst + (G0 + c0 + k_excl * diff);
+        const int len = (int)out_len;
+        for (int c = tid * 16; c < len; c += kThreads * 16) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(s.out + c);
+            if (c + 16 <= len)
+                __builtin_memcpy(out_first + c, &v, 16);
+            else
+                store_tail(out_first + c, v, len - c);
+        }
+        return true;
+    }
+
+    kp.poll();
+    __syncthreads();   // s.tile; the prefix
+    if (!kp.get(k_excl))
+        return false;
+    records(k_excl);
+    if (!p.write_output)
+        return true;
+    const int64_t out_base = G0 + c0 + k_excl * diff;
     if (!IMAGE) {
         // my pieces straight to HBM: runs as 16-byte stores with the last one ending at
         // the run's end (or one exact tail), replacements from registers
@@ -1871,32 +1929,6 @@ __device__ __forceinline__ void bf_emit_staged(const ReplaceParams &p, int64_t g
             M &= M - 1;
             if (j >= jend)
                 break;
-        }
-    } else if (out_len + 16 <= kBfOutCap && r <= kRepLds) {
-        // my pieces into the LDS image, then the workgroup stores the image
-        int oo = (int)o;
-        for (;;) {
-            const int qk = M ? (int)__builtin_ctzll(M) : jend;
-            bf_copy_run(s.out + oo, s_src + j, qk - j);
-            oo += qk - j;
-            if (qk >= jend)
-                break;
-            bf_put_rep_lds(s.out + oo, rp, rep_v, s.rep, r);
-            oo += r;
-            j = qk + L;
-            M &= M - 1;
-            if (j >= jend)
-                break;
-        }
-        __syncthreads();
-        uint8_t *const out_first = p.dst + out_base;
-        const int len = (int)out_len;
-        for (int c = tid * 16; c < len; c += kThreads * 16) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(s.out + c);
-            if (c + 16 <= len)
-                __builtin_memcpy(out_first + c, &v, 16);
-            else
-                store_tail(out_first + c, v, len - c);
         }
     } else {
         // a group whose output does not fit the image (or a long replacement): every
@@ -1920,6 +1952,7 @@ __device__ __forceinline__ void bf_emit_staged(const ReplaceParams &p, int64_t g
                 break;
         }
     }
+    return true;
 }
 
 // The LDS arrays of a border-free emit workgroup, declared in the kernel.
@@ -1969,7 +2002,8 @@ bf_emit_kernel(ReplaceParams p, int64_t first_group, int64_t groups)
         // the next group's source (the last iteration loads its own group again)
         bf_prefetch<true>(p, g + gridDim.x < g_end ? g + gridDim.x : g, tid, stage);
         __syncthreads();
-        bf_emit_staged<FULL4, true>(p, g, k_excl, lds, rep_v, rp);
+        KnownK kk{k_excl};
+        bf_emit_staged<FULL4, true>(p, g, kk, lds, rep_v, rp);
     }
 }
 
@@ -2009,13 +2043,14 @@ bf_emit_direct_kernel(ReplaceParams p, int64_t first_group)
     __syncthreads();
     const uint4 rep_v = lds_load16(s_rep);
     const BfRep rp = bf_rep_parts(rep_v, p.r);
-    bf_emit_staged<FULL4, false>(p, g, k_excl, lds, rep_v, rp);
+    KnownK kk{k_excl};
+    bf_emit_staged<FULL4, false>(p, g, kk, lds, rep_v, rp);
 }
 
 // ---- single-pass border-free sweep -------------------------------------------------
 // One persistent launch does the whole pass.  Workgroup 0 is the scanner: its first
-// wave turns the groups' kept counts into exclusive prefixes, 64 groups per step with
-// 16 steps of loads in flight.  Every other workgroup is a worker w < W = grid - 1 and
+// wave turns the groups' kept counts into exclusive prefixes, polling 16 blocks of 64
+// groups at a time.  Every other workgroup is a worker w < W = grid - 1 and
 // owns group kW + w of round k.  Its phases alternate: the count phase of round k
 // (stage, count, publish the count) runs `ahead` rounds before the emit phase of the
 // same group (stage it again -- it was read `ahead` rounds earlier, so the read
@@ -2043,48 +2078,90 @@ __device__ __forceinline__ void st_agent(uint64_t *p, uint64_t v)
 
 constexpr uint64_t kTagMask = ~((1ull << kEpochShift) - 1);
 
+// The prefix of a persistent launch's group, waited for inside its emit (KnownK's
+// counterpart): thread 0 polls the scanner's word, starting from one it loaded
+// earlier, and hands the value (or a give-up) to the workgroup through LDS.
+struct LateK {
+    const SweepArgs *a;
+    uint64_t tag;
+    int64_t g;
+    uint64_t pv;          // thread 0: the group's prefix word as loaded earlier
+    int64_t *s_k;
+    int *s_abort;
+    __device__ __forceinline__ void poll()
+    {
+        if (threadIdx.x != 0)
+            return;
+        uint64_t v = pv;
+        int spins = 0;
+        while ((v & kTagMask) != tag) {
+            if (++spins > a->spin_limit || (spins % 64 == 0 && ld_agent(a->flag) == (tag | 1))) {
+                st_agent(a->flag, tag | 1);
+                *s_abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            v = ld_agent(a->pref + g);
+        }
+        *s_k = (int64_t)(v & ~kTagMask);
+    }
+    __device__ __forceinline__ bool get(int64_t &out) const
+    {
+        if (*s_abort)
+            return false;
+        out = *s_k;
+        return true;
+    }
+};
+
 __device__ __forceinline__ void bf_scanner(const ReplaceParams &p, const SweepArgs &a, int lane, uint64_t tag,
                                            int64_t G)
 {
+    // one poll loads the next KB blocks of 64 counts together and scans every leading
+    // block whose counts are all published, so a poll at the front of the published
+    // counts advances as far as they reach (not one block per round trip)
     constexpr int KB = 16;
     uint64_t running = 0;
     const int64_t nb = (G + kWave - 1) / kWave;
-    for (int64_t b0 = 0; b0 < nb; b0 += KB) {
+    int spins = 0;
+    for (int64_t b = 0; b < nb;) {
         uint64_t v[KB];
 #pragma unroll
         for (int i = 0; i < KB; ++i) {
-            const int64_t idx = (b0 + i) * kWave + lane;
+            const int64_t idx = (b + i) * kWave + lane;
             v[i] = ld_agent(a.agg + (idx < G ? idx : G - 1));
         }
+        int done = 0;
+        bool go = true;   // wave-uniform: every block so far was published
 #pragma unroll
         for (int i = 0; i < KB; ++i) {
-            if (b0 + i >= nb)
-                break;
-            const int64_t idx = (b0 + i) * kWave + lane;
-            uint64_t x = v[i];
-            int spins = 0;
-            for (;;) {
-                const bool ok = idx >= G || (x & kTagMask) == tag;
-                if (__builtin_amdgcn_ballot_w64(!ok) == 0)
-                    break;
-                if (++spins > a.spin_limit || (spins % 64 == 0 && ld_agent(a.flag) == (tag | 1))) {
-                    if (lane == 0)
-                        st_agent(a.flag, tag | 1);
-                    return;
-                }
-                __builtin_amdgcn_s_sleep(2);
-                if (!ok)
-                    x = ld_agent(a.agg + idx);
+            const int64_t idx = (b + i) * kWave + lane;
+            const bool ok = idx >= G || (v[i] & kTagMask) == tag;
+            go = go && b + i < nb && __builtin_amdgcn_ballot_w64(!ok) == 0;
+            if (go) {
+                const uint32_t c = idx < G ? (uint32_t)(v[i] & 0xffffffffu) : 0u;
+                const uint32_t inc = wave_incl_add_u32(c);
+                if (idx < G)
+                    st_agent(a.pref + idx, tag | (running + inc - c));
+                running += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+                done = i + 1;
             }
-            const uint32_t c = idx < G ? (uint32_t)(x & 0xffffffffu) : 0u;
-            const uint32_t inc = wave_incl_add_u32(c);
-            if (idx < G)
-                st_agent(a.pref + idx, tag | (running + inc - c));
-            running += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+        }
+        b += done;
+        if (done == 0) {
+            if (++spins > a.spin_limit || (spins % 64 == 0 && ld_agent(a.flag) == (tag | 1))) {
+                if (lane == 0)
+                    st_agent(a.flag, tag | 1);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        } else {
+            spins = 0;
         }
     }
-    if (lane == 0)
+    if (lane == 0) {
         st_agent(&p.stats->k_total, tag | running);
+    }
 }
 
 // Kept count of the staged group (all four waves), published by thread 0.
@@ -2175,11 +2252,22 @@ bf_sweep_kernel(ReplaceParams p, SweepArgs a)
     BfStage stage;
     bool emit = false;
     int64_t g = phase(0, &emit);
+    uint64_t pv = 0;       // thread 0: a prefix word loaded a phase before its emit
+    int64_t pv_g = -1;     // ... and its group
     if (IMAGE)
         bf_prefetch<false>(p, g, tid, stage);
     for (int64_t i = 0; i < phases; ++i) {
         g = phase(i, &emit);
         __syncthreads();                       // the previous phase is done with the LDS
+        if (tid == 0 && i + 1 < phases) {
+            // the next emit phase's prefix word, loaded a phase early
+            bool ne = false;
+            const int64_t gn = phase(i + 1, &ne);
+            if (ne) {
+                pv = ld_agent(a.pref + gn);
+                pv_g = gn;
+            }
+        }
         if (IMAGE) {
             // register stage: this phase's group, then the next phase's prefetch
             bf_stage_to_lds(stage, tid, s_in);
@@ -2192,30 +2280,94 @@ bf_sweep_kernel(ReplaceParams p, SweepArgs a)
         }
         if (tid == 0)
             s_halo_last = INT_MIN;
-        if (emit && tid == 0) {
-            // the group's prefix, published by the scanner
-            uint64_t v = ld_agent(a.pref + g);
-            int spins = 0;
-            while ((v & kTagMask) != tag) {
-                if (++spins > a.spin_limit || (spins % 64 == 0 && ld_agent(a.flag) == (tag | 1))) {
-                    st_agent(a.flag, tag | 1);
-                    s_abort = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-                v = ld_agent(a.pref + g);
-            }
-            s_kexcl = (int64_t)(v & ~kTagMask);
-        }
         if (!IMAGE)
             __builtin_amdgcn_s_waitcnt(0);     // the LDS DMA of the group
         __syncthreads();
-        if (s_abort)
-            return;
-        if (emit)
-            bf_emit_staged<FULL4, IMAGE>(p, g, s_kexcl, lds, rep_v, rp);
-        else
+        if (emit) {
+            LateK lk{&a, tag, g, pv_g == g ? pv : 0ull, &s_kexcl, &s_abort};
+            if (!bf_emit_staged<FULL4, IMAGE>(p, g, lk, lds, rep_v, rp))
+                return;
+        } else {
             bf_count_staged(p, g, lds, a, tag);
+        }
+    }
+}
+
+// The single launch with each group read once: a worker stages group k+1 of its own
+// sequence into the second LDS slot and counts it before it emits group k from the
+// first, so the emit phase finds its group still in LDS (count-ahead of one round).
+// Two source slots and a smaller output image (groups whose output outgrows it store
+// their pieces directly) keep three workgroups on a compute unit.  The scanner,
+// tags, bounded waits and give-up word are those of bf_sweep_kernel.
+constexpr int kKeepOutCap = kBfGroup + 2048;
+
+template <bool FULL4>
+__global__ void __launch_bounds__(kThreads)
+bf_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
+{
+    constexpr int kIn = GroupShape<kGroupTiles>::kIn;
+    __shared__ __align__(16) uint8_t s_in[2 * kIn];
+    __shared__ __align__(16) uint8_t s_out[kKeepOutCap];
+    __shared__ uint8_t s_pat[kMaxSinglePassPat];
+    __shared__ __align__(16) uint8_t s_rep[kRepLds + 64];
+    __shared__ uint32_t s_wsum[kWaves];
+    __shared__ int s_wmax[kWaves];
+    __shared__ int s_halo_last;
+    __shared__ int s_tile[4];
+    __shared__ int64_t s_kexcl;
+    __shared__ int s_abort;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    for (int i = tid; i < p.L; i += kThreads)
+        s_pat[i] = p.pat[i];
+    for (int i = tid; i < p.r && i < kRepLds; i += kThreads)
+        s_rep[i] = p.rep[i];
+    const uint64_t tag = (uint64_t)p.epoch << kEpochShift;
+    const int64_t G = p.num_groups;
+    if (blockIdx.x == 0) {
+        if (tid < kWave)
+            bf_scanner(p, a, lane, tag, G);
+        return;
+    }
+    const int64_t W = gridDim.x - 1;
+    const int64_t w = blockIdx.x - 1;
+    const int64_t R = w < G ? (G - w + W - 1) / W : 0;   // groups of this worker: kW + w, k < R
+    if (tid == 0)
+        s_abort = 0;
+    __syncthreads();
+    const uint4 rep_v = lds_load16(s_rep);
+    const BfRep rp = bf_rep_parts(rep_v, p.r);
+    if (R == 0)
+        return;
+    BfStage stage;
+    bf_prefetch<false>(p, w, tid, stage);
+    // round k: stage and count group k+1 (slot (k+1)&1), then emit group k (slot k&1);
+    // round -1 stages and counts group 0
+    uint64_t pv = 0;   // thread 0: the emitted group's prefix word, loaded before the count
+    for (int64_t k = -1; k < R; ++k) {
+        const int64_t gc = (k + 1) * W + w;            // the group counted this round
+        if (tid == 0 && k >= 0)
+            pv = ld_agent(a.pref + k * W + w);
+        if (k + 1 < R) {
+            uint8_t *const slot = s_in + ((k + 1) & 1) * kIn;
+            __syncthreads();                           // the previous round's emit is done with slot
+            bf_stage_to_lds(stage, tid, slot);
+            const int64_t gn = k + 2 < R ? gc + W : gc;
+            bf_prefetch<false>(p, gn, tid, stage);
+            __syncthreads();
+            const BfLds lc{slot, s_out, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
+            bf_count_staged(p, gc, lc, a, tag);
+        }
+        if (k < 0)
+            continue;
+        const int64_t g = k * W + w;
+        if (tid == 0)
+            s_halo_last = INT_MIN;
+        __syncthreads();                               // the count is done with s_wsum
+        const BfLds le{s_in + (k & 1) * kIn, s_out, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
+        LateK lk{&a, tag, g, pv, &s_kexcl, &s_abort};
+        if (!bf_emit_staged<FULL4, true, kKeepOutCap>(p, g, lk, le, rep_v, rp))
+            return;
     }
 }
 
@@ -3847,11 +3999,25 @@ hipError_t sweep_t(const ReplaceParams &p, const SweepArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, int spin_limit, hipStream_t s)
+template <bool FULL4>
+hipError_t sweep_keep_t(const ReplaceParams &p, const SweepArgs &a, hipStream_t s)
+{
+    const int64_t grid = std::min(p.num_groups + 1, resident_grid(bf_sweep_keep_kernel<FULL4>));
+    hipLaunchKernelGGL((bf_sweep_keep_kernel<FULL4>), dim3((unsigned)std::max<int64_t>(grid, 2)), dim3(kThreads),
+                       0, s, p, a);
+    return hipGetLastError();
+}
+
+// keep: the form that holds each group in LDS from its count to its emit (it needs
+// the output image; without one the re-staging form runs)
+hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, int spin_limit, bool keep,
+                           hipStream_t s)
 {
     const int64_t G = p.num_groups;
     SweepArgs a{sweep, sweep + G, sweep + 2 * G, ahead, spin_limit};
     const bool img = bf_image();
+    if (img && keep)
+        return p.L >= 4 ? sweep_keep_t<true>(p, a, s) : sweep_keep_t<false>(p, a, s);
     if (p.L >= 4)
         return img ? sweep_t<true, true>(p, a, s) : sweep_t<true, false>(p, a, s);
     return img ? sweep_t<false, true>(p, a, s) : sweep_t<false, false>(p, a, s);

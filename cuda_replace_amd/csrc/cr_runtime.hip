@@ -52,14 +52,16 @@ constexpr int kTuneCands[3] = {1, 2, 4};
 // recomputes candidates and takes its carry from a left halo), and the 32-byte-per-
 // thread count and emit with 1, 2 or 4 tiles per group (the count hands every
 // segment's candidate mask to the emit).
-enum BfVariant { kBfSweep = 0, kBfLean = 1, kBfTiles1 = 2, kBfTiles2 = 3, kBfTiles4 = 4, kBfVariants = 5 };
-constexpr int kBfVariantTiles[kBfVariants] = {kGroupTiles, kGroupTiles, 1, 2, 4};
+enum BfVariant { kBfSweep = 0, kBfLean = 1, kBfTiles1 = 2, kBfTiles2 = 3, kBfTiles4 = 4, kBfKeep = 5, kBfVariants = 6 };
+constexpr int kBfVariantTiles[kBfVariants] = {kGroupTiles, kGroupTiles, 1, 2, 4, kGroupTiles};
+// the single-launch forms (kBfSweep re-stages a group for its emit, kBfKeep holds it in LDS)
+constexpr bool single_launch(int v) { return v == kBfSweep || v == kBfKeep; }
 
 struct TuneEntry {
     int gt = 0;
     float ms[3] = {0, 0, 0};   // per candidate, best of three
     int variant = -1;          // border-free class: the chosen BfVariant
-    float vms[kBfVariants] = {0, 0, 0, 0, 0};
+    float vms[kBfVariants] = {0, 0, 0, 0, 0, 0};
 };
 
 struct Tuner {
@@ -385,8 +387,8 @@ int sweep_spin_limit()
 }
 
 // The form of a border-free pass (BfVariant); *tune is set when this pass should
-// measure every form.  CR_BF_VARIANT=0..4 fixes it; CR_SWEEP=0 keeps the single
-// launch out; untuned, passes of CR_SWEEP_MIN bytes (64 MiB) or more take the single
+// measure every form.  CR_BF_VARIANT=0..5 fixes it; CR_SWEEP=0 keeps the single
+// launches out; untuned, passes of CR_SWEEP_MIN bytes (64 MiB) or more take the single
 // launch and smaller ones the three launches of the same kernels.
 int choose_bf_variant(int device, int64_t n, bool *tune)
 {
@@ -396,7 +398,7 @@ int choose_bf_variant(int device, int64_t n, bool *tune)
     if (const char *e = getenv("CR_BF_VARIANT")) {
         const int v = atoi(e);
         if (v >= 0 && v < kBfVariants)
-            return v == kBfSweep && !sweep_ok ? kBfLean : v;
+            return single_launch(v) && !sweep_ok ? kBfLean : v;
     }
     const int dflt = n >= smin ? kBfSweep : kBfLean;
     const int b = size_bucket(n);
@@ -404,7 +406,7 @@ int choose_bf_variant(int device, int64_t n, bool *tune)
     std::lock_guard<std::mutex> lk(t.mtx);
     auto it = t.best.find(std::make_tuple(device, 0, b));
     if (it != t.best.end() && it->second.variant >= 0)
-        return it->second.variant == kBfSweep && !sweep_ok ? kBfLean : it->second.variant;
+        return single_launch(it->second.variant) && !sweep_ok ? kBfLean : it->second.variant;
     *tune = b > 0 && autotune_enabled();
     return dflt;
 }
@@ -591,7 +593,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     // border-free patterns: the form the tuner chose (its group size follows);
     // other patterns: the tuned group size
     const int bf_var = mode == kModeBorderFree ? choose_bf_variant(s->device, n, &tune) : -1;
-    const bool bf_lean = bf_var == kBfSweep || bf_var == kBfLean;   // 64 bytes per thread
+    const bool bf_lean = single_launch(bf_var) || bf_var == kBfLean;   // 64 bytes per thread
     const int gt = mode == kModeBorderFree ? kBfVariantTiles[bf_var]
                                            : choose_group_tiles(s->device, mode, table_size, n, &tune);
     const int64_t num_groups = groups_of(num_tiles, gt);
@@ -757,10 +759,10 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
         p.dst = dst;
         // border-free passes of a single chunk: one persistent launch when large enough
         bool swept = false;
-        if (!counted && bf_var == kBfSweep && num_chunks == 1 && ensure_sweep(s, num_groups) == hipSuccess) {
+        if (!counted && single_launch(bf_var) && num_chunks == 1 && ensure_sweep(s, num_groups) == hipSuccess) {
             if (kernel_ms)
                 CK(hipEventRecord(s->ev0, s->stream));
-            CK(launch_bf_sweep(p, s->d_sweep, sweep_ahead(), sweep_spin_limit(), s->stream));
+            CK(launch_bf_sweep(p, s->d_sweep, sweep_ahead(), sweep_spin_limit(), bf_var == kBfKeep, s->stream));
             if (kernel_ms)
                 CK(hipEventRecord(s->ev1, s->stream));
             CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * num_groups, 8, hipMemcpyDeviceToHost, s->stream));
@@ -820,7 +822,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                     q.num_groups = ng;
                     q.masks = v >= kBfTiles1 && have_mask_buf ? s->d_bitmap : nullptr;
                     float ms = kGaveUp;
-                    if (v == kBfSweep) {
+                    if (single_launch(v)) {
                         if (sweep_min_bytes() == INT64_MAX || ensure_sweep(s, ng) != hipSuccess) {
                             (void)hipGetLastError();
                             te.vms[v] = kGaveUp;
@@ -829,7 +831,8 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                         CK(next_epoch(s));
                         q.epoch = s->epoch;
                         CK(hipEventRecord(s->ev0, s->stream));
-                        CK(launch_bf_sweep(q, s->d_sweep, sweep_ahead(), sweep_spin_limit(), s->stream));
+                        CK(launch_bf_sweep(q, s->d_sweep, sweep_ahead(), sweep_spin_limit(), v == kBfKeep,
+                                           s->stream));
                         CK(hipEventRecord(s->ev1, s->stream));
                         CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * ng, 8, hipMemcpyDeviceToHost,
                                           s->stream));
@@ -1876,8 +1879,9 @@ int cr_autotune_set(const uint8_t *pat, int pat_len, size_t n, int group_tiles)
     int lt = 0;
     const int c = tune_class(pattern_mode(pat, pat_len, &lt), lt);
     // border-free patterns: 1, 2 or 4 pin the 32-byte-per-thread form with that group
-    // size, 8 the three-launch 64-byte form, 16 the single launch
-    const bool bf_codes = c == 0 && (group_tiles == 8 || group_tiles == 16);
+    // size, 8 the three-launch 64-byte form, 16 the single launch, 32 the single launch
+    // that holds each group in LDS
+    const bool bf_codes = c == 0 && (group_tiles == 8 || group_tiles == 16 || group_tiles == 32);
     if (c < 0 || !(group_tiles == 0 || group_tiles == 1 || group_tiles == 2 || group_tiles == 4 || bf_codes))
         return -1;
     int dev = 0;
@@ -1892,7 +1896,7 @@ int cr_autotune_set(const uint8_t *pat, int pat_len, size_t n, int group_tiles)
     } else if (c == 0) {
         TuneEntry te;
         te.gt = kGroupTiles;
-        te.variant = group_tiles == 16 ? kBfSweep : group_tiles == 8 ? kBfLean
+        te.variant = group_tiles == 32 ? kBfKeep : group_tiles == 16 ? kBfSweep : group_tiles == 8 ? kBfLean
                    : group_tiles == 1 ? kBfTiles1 : group_tiles == 2 ? kBfTiles2 : kBfTiles4;
         t.best[key] = te;
     } else {
@@ -1920,7 +1924,7 @@ int cr_autotune_report(char *buf, size_t cap)
         std::lock_guard<std::mutex> lk(t.mtx);
         bool first = true;
         static const char *kVariant[kBfVariants] = {"single_launch", "three_launch_64B", "tiles1_32B",
-                                                    "tiles2_32B", "tiles4_32B"};
+                                                    "tiles2_32B", "tiles4_32B", "single_launch_lds"};
         for (const auto &kv : t.best) {
             char line[512];
             const TuneEntry &te = kv.second;
@@ -1929,10 +1933,10 @@ int cr_autotune_report(char *buf, size_t cap)
                 snprintf(line, sizeof line,
                          "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"form\": \"%s\", "
                          "\"ms\": {\"single_launch\": %.4f, \"three_launch_64B\": %.4f, \"tiles1_32B\": %.4f, "
-                         "\"tiles2_32B\": %.4f, \"tiles4_32B\": %.4f}}",
+                         "\"tiles2_32B\": %.4f, \"tiles4_32B\": %.4f, \"single_launch_lds\": %.4f}}",
                          first ? "" : ", ", std::get<0>(kv.first), kClassName[std::get<1>(kv.first)],
                          kBucket[std::get<2>(kv.first)], kVariant[te.variant], f(te.vms[0]), f(te.vms[1]),
-                         f(te.vms[2]), f(te.vms[3]), f(te.vms[4]));
+                         f(te.vms[2]), f(te.vms[3]), f(te.vms[4]), f(te.vms[5]));
             } else {
                 snprintf(line, sizeof line,
                          "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"group_tiles\": %d, "

@@ -225,7 +225,8 @@ class CudaReplaceLib:
         """Pins the tiles per group (1, 2 or 4; 0 unpins) for pat's pattern class and
         the size bucket of n on the current device.  For a border-free pattern 1/2/4
         pin the 32-byte-per-thread form with that group size, 8 the three-launch
-        64-byte form and 16 the single launch."""
+        64-byte form, 16 the single launch and 32 the single launch that holds each
+        group in LDS."""
         rc = int(self.lib.cr_autotune_set(_u8(pat), len(pat), int(n), int(group_tiles)))
         if rc != 0:
             raise RuntimeError(f"cr_autotune_set failed rc={rc}")

@@ -119,20 +119,23 @@ CR_API int cr_batch_fusable(const uint8_t *const *pats, const int *pat_lens,
 
 /* Autotuner.  Carry-table classes (2/4/8/16 states): the tiles per workgroup group
  * (1, 2 or 4 tiles of 8 KiB).  Border-free patterns: the form of the pass -- one
- * persistent launch (64 source bytes per thread, counts ahead of the emit), the
- * same kernels as three launches, or the 32-byte-per-thread count and emit with 1,
- * 2 or 4 tiles per group.  The first pass of a class in an input size bucket
+ * persistent launch (64 source bytes per thread, counts ahead of the emit) that
+ * reads each group again for its emit, or one that holds it in LDS from its count
+ * to its emit, the same kernels as three launches, or the 32-byte-per-thread count
+ * and emit with 1, 2 or 4 tiles per group.  The first pass of a class in an input size bucket
  * (under 4 MiB, 4-32 MiB, 32-256 MiB, 256 MiB-2 GiB, larger) on a device times
  * each candidate after its normal pass and keeps the fastest; inputs under 4 MiB
  * keep the default unless pinned.  CR_GROUP_TILES=1|2|4 fixes the tables' choice,
- * CR_BF_VARIANT=0..4 the border-free form (0 one launch, 1 three launches, 2/3/4
- * 32-byte threads with 1/2/4 tiles), CR_AUTOTUNE=0 disables measuring.  No
+ * CR_BF_VARIANT=0..5 the border-free form (0 one launch, 1 three launches, 2/3/4
+ * 32-byte threads with 1/2/4 tiles, 5 one launch holding groups in LDS),
+ * CR_AUTOTUNE=0 disables measuring.  No
  * counterpart in the reference.
  *
  * cr_autotune_set pins the choice for the class of pattern pat and the bucket of n
  * on the current device; 0 unpins.  group_tiles 1, 2 or 4 is a group size (for a
  * border-free pattern: the 32-byte form with that group size); for a border-free
- * pattern 8 pins the three-launch 64-byte form and 16 the single launch.  Returns
+ * pattern 8 pins the three-launch 64-byte form, 16 the single launch and 32 the
+ * single launch that holds groups in LDS.  Returns
  * 0, or -1 for a pattern outside the tuned classes (empty, or the bitmap path) or a
  * bad value. */
 CR_API int cr_autotune_set(const uint8_t *pat, int pat_len, size_t n, int group_tiles);

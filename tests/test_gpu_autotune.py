@@ -1,7 +1,7 @@
 """Autotuner on the GPU.  Every carry-table group size (1, 2 or 4 tiles of 8 KiB per
 workgroup) and every form of a border-free pass (the 32-byte-per-thread count and
 emit with 1, 2 or 4 tiles per group, the 64-byte three-launch pass, the single
-launch) must give the oracle's output around the tile and group boundaries of each
+launch re-reading or holding each group) must give the oracle's output around the tile and group boundaries of each
 size, plain and seeded with prefix queries; and the first pass in a tuned size
 bucket must record a choice with a measured time for every candidate (the default
 unless another is more than 3% faster), and still return the exact output."""
@@ -30,11 +30,11 @@ def pinned(lib):
 
 
 def _pinnable(lib, pat, gt):
-    """1/2/4 pin any tuned class; 8 and 16 (the 64-byte forms) only border-free ones."""
+    """1/2/4 pin any tuned class; 8, 16 and 32 (the 64-byte forms) only border-free ones."""
     return gt in (1, 2, 4) or lib.pattern_mode(pat) == 1
 
 
-@pytest.mark.parametrize("gt", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("gt", [1, 2, 4, 8, 16, 32])
 def test_every_group_size_exact(lib, pinned, gt):
     rng = random.Random(60 + gt)
     for pat, rep in PATS:
@@ -48,7 +48,7 @@ def test_every_group_size_exact(lib, pinned, gt):
         assert lib.unified(src, pat, rep) == src.replace(pat, rep), (gt, pat, "all-a")
 
 
-@pytest.mark.parametrize("gt", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("gt", [1, 2, 4, 8, 16, 32])
 def test_group_size_seeded_and_query(lib, pinned, gt):
     rng = random.Random(70 + gt)
     src = _src(rng, 50_000)
@@ -86,10 +86,10 @@ def test_first_tuned_pass_measures_every_candidate(lib, pinned):
     else:
         assert ms[str(t["group_tiles"])] == best and best < 0.97 * ms["2"]
     b = classes["border_free"]
-    forms = ("single_launch", "three_launch_64B", "tiles1_32B", "tiles2_32B", "tiles4_32B")
+    forms = ("single_launch", "three_launch_64B", "tiles1_32B", "tiles2_32B", "tiles4_32B", "single_launch_lds")
     assert b["size"] == "4-32MiB" and b["form"] in forms
     ms = b["ms"]
-    assert all(ms[k] > 0 or (k == "single_launch" and ms[k] == -1) for k in forms)   # -1: gave up
+    assert all(ms[k] > 0 or (k.startswith("single_launch") and ms[k] == -1) for k in forms)   # -1: gave up
     timed = {k: v for k, v in ms.items() if v > 0}
     dflt = "three_launch_64B"          # the default under 64 MiB
     if b["form"] == dflt:

@@ -3,7 +3,9 @@
 By default the library runs a border-free pass as one persistent launch (count
 phases `ahead` rounds before the emit phases, a scanner workgroup publishing each
 group's prefix) only from 64 MiB on; CR_SWEEP_MIN=0 makes every border-free pass a
-sweep, so the edge cases (one group, fewer groups than workers, tile and group
+sweep.  Every test runs both single-launch forms (CR_BF_VARIANT=0 re-stages a group
+for its emit, 5 holds it in LDS one round from its count to its emit), so the edge
+cases (one group, fewer groups than workers, tile and group
 boundaries, seeds, every pattern length class of the count's masked SADs) run here.
 CR_SWEEP_AHEAD varies the count-ahead distance, and CR_SWEEP_SPIN=1 makes the
 bounded waits give up at once, which must take the host's fallback to the three
@@ -23,10 +25,12 @@ def say(*a):
     print(*a, flush=True)
 
 
-@pytest.fixture
-def sweep_env():
-    saved = {k: os.environ.get(k) for k in ("CR_SWEEP", "CR_SWEEP_MIN", "CR_SWEEP_AHEAD", "CR_SWEEP_SPIN")}
+@pytest.fixture(params=["0", "5"], ids=["restage", "hold"])
+def sweep_env(request):
+    saved = {k: os.environ.get(k) for k in ("CR_SWEEP", "CR_SWEEP_MIN", "CR_SWEEP_AHEAD", "CR_SWEEP_SPIN",
+                                             "CR_BF_VARIANT")}
     os.environ["CR_SWEEP_MIN"] = "0"
+    os.environ["CR_BF_VARIANT"] = request.param
     os.environ.pop("CR_SWEEP", None)
     yield os.environ
     for k, v in saved.items():
