@@ -386,10 +386,20 @@ int sweep_spin_limit()
     return v < 1 ? 1 : v;
 }
 
+// The untuned form of a border-free pass of n bytes: from 256 MiB the single launch
+// holding groups in LDS, from CR_SWEEP_MIN (64 MiB) the single launch re-staging
+// them, below it the three launches of the same kernels.
+int default_bf_variant(int64_t n)
+{
+    const int64_t smin = sweep_min_bytes();
+    if (n < smin)
+        return kBfLean;
+    return n >= (256ll << 20) ? kBfKeep : kBfSweep;
+}
+
 // The form of a border-free pass (BfVariant); *tune is set when this pass should
 // measure every form.  CR_BF_VARIANT=0..5 fixes it; CR_SWEEP=0 keeps the single
-// launches out; untuned, passes of CR_SWEEP_MIN bytes (64 MiB) or more take the single
-// launch and smaller ones the three launches of the same kernels.
+// launches out; otherwise the tuned choice for the size bucket, or the default.
 int choose_bf_variant(int device, int64_t n, bool *tune)
 {
     *tune = false;
@@ -400,7 +410,7 @@ int choose_bf_variant(int device, int64_t n, bool *tune)
         if (v >= 0 && v < kBfVariants)
             return single_launch(v) && !sweep_ok ? kBfLean : v;
     }
-    const int dflt = n >= smin ? kBfSweep : kBfLean;
+    const int dflt = default_bf_variant(n);
     const int b = size_bucket(n);
     Tuner &t = tuner();
     std::lock_guard<std::mutex> lk(t.mtx);
@@ -855,7 +865,7 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                     te.vms[v] = sweep == 0 || ms < te.vms[v] ? ms : te.vms[v];
                 }
             }
-            const int dflt = n >= sweep_min_bytes() ? kBfSweep : kBfLean;
+            const int dflt = default_bf_variant(n);
             te.variant = dflt;
             float best = te.vms[dflt];
             for (int v = 0; v < kBfVariants; ++v) {
