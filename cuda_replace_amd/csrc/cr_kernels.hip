@@ -2063,9 +2063,27 @@ struct SweepArgs {
     uint64_t *agg;        // per group: epoch << 48 | kept count
     uint64_t *pref;       // per group: epoch << 48 | kept starts before the group
     uint64_t *flag;       // epoch << 48 | 1 once a wait has given up
+    uint64_t *arrive;     // epoch << 48 | 1 once the grid's last workgroup has started
     int32_t ahead;        // rounds the count phase runs ahead of the emit phase
     int32_t spin_limit;   // polls before a wait gives up
 };
+
+// Workgroups start in order of their index, so once the last one has started every
+// workgroup of the grid is (or was) resident.  A wait still unanswered after this
+// many polls while the last workgroup has not started gives up at once: another
+// kernel holds part of the device, and the grid cannot finish until this one ends.
+constexpr int kResidentPolls = 256;
+
+__device__ __forceinline__ bool sweep_should_give_up(const SweepArgs &a, uint64_t tag, int spins)
+{
+    if (spins > a.spin_limit)
+        return true;
+    if (spins % 64 != 0)
+        return false;
+    return __hip_atomic_load(a.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (tag | 1) ||
+           (spins >= kResidentPolls &&
+            __hip_atomic_load(a.arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != (tag | 1));
+}
 
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
 {
@@ -2095,7 +2113,7 @@ struct LateK {
         uint64_t v = pv;
         int spins = 0;
         while ((v & kTagMask) != tag) {
-            if (++spins > a->spin_limit || (spins % 64 == 0 && ld_agent(a->flag) == (tag | 1))) {
+            if (sweep_should_give_up(*a, tag, ++spins)) {
                 st_agent(a->flag, tag | 1);
                 *s_abort = 1;
                 break;
@@ -2149,7 +2167,7 @@ __device__ __forceinline__ void bf_scanner(const ReplaceParams &p, const SweepAr
         }
         b += done;
         if (done == 0) {
-            if (++spins > a.spin_limit || (spins % 64 == 0 && ld_agent(a.flag) == (tag | 1))) {
+            if (sweep_should_give_up(a, tag, ++spins)) {
                 if (lane == 0)
                     st_agent(a.flag, tag | 1);
                 return;
@@ -2215,6 +2233,8 @@ bf_sweep_kernel(ReplaceParams p, SweepArgs a)
         s_rep[i] = p.rep[i];
     const uint64_t tag = (uint64_t)p.epoch << kEpochShift;
     const int64_t G = p.num_groups;
+    if (tid == 0 && blockIdx.x == gridDim.x - 1)
+        st_agent(a.arrive, tag | 1);
     if (blockIdx.x == 0) {
         if (tid < kWave)
             bf_scanner(p, a, lane, tag, G);
@@ -2324,6 +2344,8 @@ bf_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
         s_rep[i] = p.rep[i];
     const uint64_t tag = (uint64_t)p.epoch << kEpochShift;
     const int64_t G = p.num_groups;
+    if (tid == 0 && blockIdx.x == gridDim.x - 1)
+        st_agent(a.arrive, tag | 1);
     if (blockIdx.x == 0) {
         if (tid < kWave)
             bf_scanner(p, a, lane, tag, G);
@@ -4014,7 +4036,7 @@ hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, i
                            hipStream_t s)
 {
     const int64_t G = p.num_groups;
-    SweepArgs a{sweep, sweep + G, sweep + 2 * G, ahead, spin_limit};
+    SweepArgs a{sweep, sweep + G, sweep + 2 * G, sweep + 2 * G + 1, ahead, spin_limit};
     const bool img = bf_image();
     if (img && keep)
         return p.L >= 4 ? sweep_keep_t<true>(p, a, s) : sweep_keep_t<false>(p, a, s);
@@ -4023,7 +4045,7 @@ hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, i
     return img ? sweep_t<false, true>(p, a, s) : sweep_t<false, false>(p, a, s);
 }
 
-int64_t sweep_words(int64_t groups) { return 2 * groups + 1; }
+int64_t sweep_words(int64_t groups) { return 2 * groups + 2; }
 
 hipError_t launch_bf_emit(const ReplaceParams &p, int64_t first_group, int64_t groups, hipStream_t s)
 {

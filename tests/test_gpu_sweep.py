@@ -115,3 +115,38 @@ def test_sweep_off_matches(lib, sweep_env):
     sweep_env.pop("CR_SWEEP")
     b = lib.unified(src, b"abcd", b"!")
     assert a == b == src.replace(b"abcd", b"!")
+
+
+def test_concurrent_single_launches(lib, sweep_env):
+    """Two sessions on two threads run 32 MiB single-launch passes at the same time.
+    The grid of each is sized to the whole device, so when both are in flight neither
+    may be resident as a whole; a wait then gives up after a few hundred polls (the
+    grid's last workgroup has not started) and the pass takes the three launches.
+    Every output must be exact."""
+    import threading
+    import time
+    src = text(1 << 16, 11) * 512                      # 32 MiB: 2048 groups per pass
+    pairs = [(b"abcd", b"0123456789AB"), (b"ab", b"_")]
+    wants = [src.replace(p, r) for p, r in pairs]
+    errors = []
+
+    def work(i):
+        pat, rep = pairs[i]
+        try:
+            with Session(lib, src) as s:
+                for k in range(6):
+                    s.reset(src)
+                    s.apply(pat, rep)
+                    if s.result() != wants[i]:
+                        errors.append((i, k))
+        except Exception as e:   # reported below
+            errors.append(e)
+
+    t0 = time.time()
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    say("concurrent single launches: %.2f s" % (time.time() - t0))
+    assert not errors, errors
