@@ -1647,6 +1647,48 @@ __device__ __forceinline__ uint32_t bf_count_seg(const ReplaceParams &p, const u
     return (uint32_t)__builtin_popcountll(bf_cand64<(NQ > 1)>(p, s_seg, s_pat, gpos));
 }
 
+// The candidate mask of the 64 positions whose bytes start at s_seg (all in range),
+// from the same masked quad SADs as bf_count64_qsad: bit i set where the window
+// at position i sums to 0.  (0x80008000 - w) has bit 15 set exactly where the low
+// 16-bit sum of w is 0 and bit 31 where the high one is.
+template <int NQ>
+__device__ __forceinline__ uint64_t bf_mask64_qsad(const ReplaceParams &p, const uint8_t *s_seg)
+{
+    uint32_t d[20];
+    const u32x4 *q4 = reinterpret_cast<const u32x4 *>(s_seg);
+    const uint32_t x = p.qx;
+#pragma unroll
+    for (int k = 0; k < (16 + NQ + 3) / 4; ++k) {
+        const u32x4 v = q4[k];
+        d[4 * k] = v.x ^ x;
+        d[4 * k + 1] = v.y ^ x;
+        d[4 * k + 2] = v.z ^ x;
+        d[4 * k + 3] = v.w ^ x;
+    }
+    uint32_t half[2] = {0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            acc = __builtin_amdgcn_mqsad_pk_u16_u8(((uint64_t)d[k + q + 1] << 32) | d[k + q], p.qref[q], acc);
+        const uint32_t tl = (0x80008000u - (uint32_t)acc) & 0x80008000u;
+        const uint32_t th = (0x80008000u - (uint32_t)(acc >> 32)) & 0x80008000u;
+        const uint32_t nib = ((tl >> 15) | (tl >> 30) | (th >> 13) | (th >> 28)) & 0xfu;
+        half[k / 8] |= nib << (4 * (k % 8));
+    }
+    return ((uint64_t)half[1] << 32) | half[0];
+}
+
+template <int NQ>
+__device__ __forceinline__ uint64_t bf_mask_seg(const ReplaceParams &p, const uint8_t *s_seg, const uint8_t *s_pat,
+                                                int64_t gpos)
+{
+    if (gpos >= p.floor && gpos + 63 <= p.n - p.L)
+        return bf_mask64_qsad<NQ>(p, s_seg);
+    return bf_cand64<(NQ > 1)>(p, s_seg, s_pat, gpos);
+}
+
 template <bool FULL4>
 __global__ void __launch_bounds__(kThreads)
 bf_count_kernel(ReplaceParams p, int64_t first_group)
@@ -1770,7 +1812,7 @@ struct KnownK {
 // before the LDS is reused); false if the prefix wait gave up.
 template <bool FULL4, bool IMAGE, int CAP = kBfOutCap, class KP>
 __device__ __forceinline__ bool bf_emit_staged(const ReplaceParams &p, int64_t g, KP &kp, const BfLds &s,
-                                               uint4 rep_v, const BfRep &rp)
+                                               uint4 rep_v, const BfRep &rp, const uint64_t *m_in = nullptr)
 {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -1804,7 +1846,9 @@ __device__ __forceinline__ bool bf_emit_staged(const ReplaceParams &p, int64_t g
     }
     const int seg = tid * kBfSeg;
     const int64_t gpos = G0 + seg;
-    const uint64_t m = gpos < n ? bf_cand64<FULL4>(p, s.in + kHalo + seg, s.pat, gpos) : 0ull;
+    // the candidate mask, unless the caller has it (the held-group launch computed it
+    // in the group's count phase)
+    const uint64_t m = m_in ? *m_in : gpos < n ? bf_cand64<FULL4>(p, s.in + kHalo + seg, s.pat, gpos) : 0ull;
     const int k_t = __builtin_popcountll(m);
     const int lq = m ? seg + 63 - (int)__builtin_clzll(m) : INT_MIN;   // last kept start (group-relative)
     const uint32_t kinc = wave_incl_add_u32((uint32_t)k_t);
@@ -2210,6 +2254,36 @@ __device__ __forceinline__ void bf_count_staged(const ReplaceParams &p, int64_t 
         st_agent(a.agg + g, tag | (uint64_t)(s.wsum[0] + s.wsum[1] + s.wsum[2] + s.wsum[3]));
 }
 
+// Count phase of the held-group launch: the staged group's candidate masks (returned
+// per thread for the group's emit) and its kept count, published by thread 0.
+__device__ __forceinline__ uint64_t bf_mask_count_staged(const ReplaceParams &p, int64_t g, const BfLds &s,
+                                                         const SweepArgs &a, uint64_t tag)
+{
+    const int tid = threadIdx.x;
+    const int64_t gpos = g * (int64_t)kBfGroup + (int64_t)tid * kBfSeg;
+    uint64_t m = 0;
+    if (gpos < p.n) {
+        const uint8_t *seg = s.in + kHalo + tid * kBfSeg;
+        if (p.L <= 4)
+            m = bf_mask_seg<1>(p, seg, s.pat, gpos);
+        else if (p.L <= 8)
+            m = bf_mask_seg<2>(p, seg, s.pat, gpos);
+        else if (p.L <= 12)
+            m = bf_mask_seg<3>(p, seg, s.pat, gpos);
+        else if (p.L <= 16)
+            m = bf_mask_seg<4>(p, seg, s.pat, gpos);
+        else
+            m = bf_cand64<true>(p, seg, s.pat, gpos);
+    }
+    const uint32_t cnt = wave_incl_add_u32((uint32_t)__builtin_popcountll(m));
+    if ((tid & (kWave - 1)) == kWave - 1)
+        s.wsum[tid / kWave] = cnt;
+    __syncthreads();
+    if (tid == 0)
+        st_agent(a.agg + g, tag | (uint64_t)(s.wsum[0] + s.wsum[1] + s.wsum[2] + s.wsum[3]));
+    return m;
+}
+
 template <bool FULL4, bool IMAGE>
 __global__ void __launch_bounds__(kThreads)
 bf_sweep_kernel(ReplaceParams p, SweepArgs a)
@@ -2366,6 +2440,7 @@ bf_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
     // round k: stage and count group k+1 (slot (k+1)&1), then emit group k (slot k&1);
     // round -1 stages and counts group 0
     uint64_t pv = 0;   // thread 0: the emitted group's prefix word, loaded before the count
+    uint64_t m_cur = 0, m_next = 0;   // candidate masks of the emitted and the counted group
     for (int64_t k = -1; k < R; ++k) {
         const int64_t gc = (k + 1) * W + w;            // the group counted this round
         if (tid == 0 && k >= 0)
@@ -2378,18 +2453,21 @@ bf_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
             bf_prefetch<false>(p, gn, tid, stage);
             __syncthreads();
             const BfLds lc{slot, s_out, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
-            bf_count_staged(p, gc, lc, a, tag);
+            m_next = bf_mask_count_staged(p, gc, lc, a, tag);
         }
-        if (k < 0)
+        if (k < 0) {
+            m_cur = m_next;
             continue;
+        }
         const int64_t g = k * W + w;
         if (tid == 0)
             s_halo_last = INT_MIN;
         __syncthreads();                               // the count is done with s_wsum
         const BfLds le{s_in + (k & 1) * kIn, s_out, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
         LateK lk{&a, tag, g, pv, &s_kexcl, &s_abort};
-        if (!bf_emit_staged<FULL4, true, kKeepOutCap>(p, g, lk, le, rep_v, rp))
+        if (!bf_emit_staged<FULL4, true, kKeepOutCap>(p, g, lk, le, rep_v, rp, &m_cur))
             return;
+        m_cur = m_next;
     }
 }
 
