@@ -1791,6 +1791,43 @@ struct BfLds {
     int *tile;           // 4
 };
 
+// Carry into the group at G0: an occurrence starting in the L-1 bytes left of it
+// (kept unless below the floor, since occurrences never overlap), its start relative
+// to G0 raised into *halo_last by the threads below L-1.
+__device__ __forceinline__ void bf_halo_carry(const ReplaceParams &p, const uint8_t *s_in, const uint8_t *s_pat,
+                                              int64_t G0, int *halo_last)
+{
+    const int tid = threadIdx.x;
+    if (G0 > 0 && tid < p.L - 1) {
+        const int qq = -1 - tid;
+        const int64_t gp = G0 + qq;
+        if (gp >= p.floor && gp + p.L <= p.n) {
+            const uint8_t *sp = s_in + kHalo + qq;
+            bool ok = true;
+            for (int k = 0; k < p.L; ++k) {
+                if (sp[k] != s_pat[k]) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok)
+                atomicMax(halo_last, qq);
+        }
+    }
+}
+
+// What a group's count phase can hand its emit (the held-group launch): each thread's
+// candidate mask and the wave-inclusive scans of kept counts and last kept starts, and
+// in LDS the per-wave totals and the carry-in occurrence of the group's left halo.
+struct BfPre {
+    uint64_t m;
+    uint32_t kinc;
+    int linc;
+    const uint32_t *wsum;
+    const int *wmax;
+    const int *halo_last;
+};
+
 // The kept starts before a group, for an emit that has them at its start (KnownK)
 // or waits for them only where the output address is first needed (a persistent
 // launch's prefix word, polled by thread 0; see LateK beside the sweep kernels).
@@ -1812,7 +1849,7 @@ struct KnownK {
 // before the LDS is reused); false if the prefix wait gave up.
 template <bool FULL4, bool IMAGE, int CAP = kBfOutCap, class KP>
 __device__ __forceinline__ bool bf_emit_staged(const ReplaceParams &p, int64_t g, KP &kp, const BfLds &s,
-                                               uint4 rep_v, const BfRep &rp, const uint64_t *m_in = nullptr)
+                                               uint4 rep_v, const BfRep &rp, const BfPre *pre = nullptr)
 {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -1826,47 +1863,43 @@ __device__ __forceinline__ bool bf_emit_staged(const ReplaceParams &p, int64_t g
     const int valid = (int)((n - G0) < kBfGroup ? (n - G0) : kBfGroup);
     const bool last_group = G0 + kBfGroup >= n;
 
-    // carry into the group: an occurrence in the L-1 bytes left of it (kept unless
-    // below the floor, since occurrences never overlap)
-    if (G0 > 0 && tid < L - 1) {
-        const int qq = -1 - tid;
-        const int64_t gp = G0 + qq;
-        if (gp >= p.floor && gp + L <= n) {
-            const uint8_t *sp = s.in + kHalo + qq;
-            bool ok = true;
-            for (int k = 0; k < L; ++k) {
-                if (sp[k] != s.pat[k]) {
-                    ok = false;
-                    break;
-                }
-            }
-            if (ok)
-                atomicMax(s.halo_last, qq);
-        }
-    }
     const int seg = tid * kBfSeg;
     const int64_t gpos = G0 + seg;
-    // the candidate mask, unless the caller has it (the held-group launch computed it
-    // in the group's count phase)
-    const uint64_t m = m_in ? *m_in : gpos < n ? bf_cand64<FULL4>(p, s.in + kHalo + seg, s.pat, gpos) : 0ull;
+    uint64_t m;
+    uint32_t kinc;
+    int linc;
+    const uint32_t *wsum = s.wsum;
+    const int *wmax = s.wmax;
+    const int *halo_last = s.halo_last;
+    if (pre) {
+        m = pre->m;
+        kinc = pre->kinc;
+        linc = pre->linc;
+        wsum = pre->wsum;
+        wmax = pre->wmax;
+        halo_last = pre->halo_last;
+    } else {
+        bf_halo_carry(p, s.in, s.pat, G0, s.halo_last);
+        m = gpos < n ? bf_cand64<FULL4>(p, s.in + kHalo + seg, s.pat, gpos) : 0ull;
+        kinc = wave_incl_add_u32((uint32_t)__builtin_popcountll(m));
+        linc = wave_incl_max_i32(m ? seg + 63 - (int)__builtin_clzll(m) : INT_MIN);
+        if (lane == kWave - 1) {
+            s.wsum[wave] = kinc;
+            s.wmax[wave] = linc;
+        }
+        __syncthreads();
+    }
     const int k_t = __builtin_popcountll(m);
     const int lq = m ? seg + 63 - (int)__builtin_clzll(m) : INT_MIN;   // last kept start (group-relative)
-    const uint32_t kinc = wave_incl_add_u32((uint32_t)k_t);
-    const int linc = wave_incl_max_i32(lq);
-    if (lane == kWave - 1) {
-        s.wsum[wave] = kinc;
-        s.wmax[wave] = linc;
-    }
-    __syncthreads();
-    const int c0 = *s.halo_last == INT_MIN ? 0 : *s.halo_last + L;   // leading group bytes covered
+    const int c0 = *halo_last == INT_MIN ? 0 : *halo_last + L;   // leading group bytes covered
     uint32_t kbefore = kinc - (uint32_t)k_t;
     int blk_prev = wave_shr1_i32(linc, INT_MIN);   // last kept start before my segment
     uint32_t ktot = 0;
     int gl = INT_MIN;   // last kept start of the group
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
-        const uint32_t ws = s.wsum[w];
-        const int wm = s.wmax[w];
+        const uint32_t ws = wsum[w];
+        const int wm = wmax[w];
         if (w < wave) {
             kbefore += ws;
             blk_prev = max(blk_prev, wm);
@@ -2254,34 +2287,51 @@ __device__ __forceinline__ void bf_count_staged(const ReplaceParams &p, int64_t 
         st_agent(a.agg + g, tag | (uint64_t)(s.wsum[0] + s.wsum[1] + s.wsum[2] + s.wsum[3]));
 }
 
-// Count phase of the held-group launch: the staged group's candidate masks (returned
-// per thread for the group's emit) and its kept count, published by thread 0.
-__device__ __forceinline__ uint64_t bf_mask_count_staged(const ReplaceParams &p, int64_t g, const BfLds &s,
-                                                         const SweepArgs &a, uint64_t tag)
+// Count phase of the held-group launch over the group staged at s_in: everything
+// its emit needs before the group's prefix (candidate masks, the wave scans of kept
+// counts and last kept starts, per-wave totals in wsum/wmax, the halo carry in
+// *halo_last, which thread 0 set to INT_MIN before the barrier ahead of this call),
+// and the kept count, published by thread 0.
+__device__ __forceinline__ BfPre bf_pre_count_staged(const ReplaceParams &p, int64_t g, const uint8_t *s_in,
+                                                    const uint8_t *s_pat, const SweepArgs &a, uint64_t tag,
+                                                    uint32_t *wsum, int *wmax, int *halo_last)
 {
     const int tid = threadIdx.x;
-    const int64_t gpos = g * (int64_t)kBfGroup + (int64_t)tid * kBfSeg;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int64_t G0 = g * (int64_t)kBfGroup;
+    const int seg = tid * kBfSeg;
+    const int64_t gpos = G0 + seg;
+    bf_halo_carry(p, s_in, s_pat, G0, halo_last);
     uint64_t m = 0;
     if (gpos < p.n) {
-        const uint8_t *seg = s.in + kHalo + tid * kBfSeg;
+        const uint8_t *sg = s_in + kHalo + seg;
         if (p.L <= 4)
-            m = bf_mask_seg<1>(p, seg, s.pat, gpos);
+            m = bf_mask_seg<1>(p, sg, s_pat, gpos);
         else if (p.L <= 8)
-            m = bf_mask_seg<2>(p, seg, s.pat, gpos);
+            m = bf_mask_seg<2>(p, sg, s_pat, gpos);
         else if (p.L <= 12)
-            m = bf_mask_seg<3>(p, seg, s.pat, gpos);
+            m = bf_mask_seg<3>(p, sg, s_pat, gpos);
         else if (p.L <= 16)
-            m = bf_mask_seg<4>(p, seg, s.pat, gpos);
+            m = bf_mask_seg<4>(p, sg, s_pat, gpos);
         else
-            m = bf_cand64<true>(p, seg, s.pat, gpos);
+            m = bf_cand64<true>(p, sg, s_pat, gpos);
     }
-    const uint32_t cnt = wave_incl_add_u32((uint32_t)__builtin_popcountll(m));
-    if ((tid & (kWave - 1)) == kWave - 1)
-        s.wsum[tid / kWave] = cnt;
+    BfPre r;
+    r.m = m;
+    r.kinc = wave_incl_add_u32((uint32_t)__builtin_popcountll(m));
+    r.linc = wave_incl_max_i32(m ? seg + 63 - (int)__builtin_clzll(m) : INT_MIN);
+    r.wsum = wsum;
+    r.wmax = wmax;
+    r.halo_last = halo_last;
+    if (lane == kWave - 1) {
+        wsum[wave] = r.kinc;
+        wmax[wave] = r.linc;
+    }
     __syncthreads();
     if (tid == 0)
-        st_agent(a.agg + g, tag | (uint64_t)(s.wsum[0] + s.wsum[1] + s.wsum[2] + s.wsum[3]));
-    return m;
+        st_agent(a.agg + g, tag | (uint64_t)(wsum[0] + wsum[1] + wsum[2] + wsum[3]));
+    return r;
 }
 
 template <bool FULL4, bool IMAGE>
@@ -2404,9 +2454,9 @@ bf_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
     __shared__ __align__(16) uint8_t s_out[kKeepOutCap];
     __shared__ uint8_t s_pat[kMaxSinglePassPat];
     __shared__ __align__(16) uint8_t s_rep[kRepLds + 64];
-    __shared__ uint32_t s_wsum[kWaves];
-    __shared__ int s_wmax[kWaves];
-    __shared__ int s_halo_last;
+    __shared__ uint32_t s_wsum[2][kWaves];   // per slot: the count phase's totals for the emit
+    __shared__ int s_wmax[2][kWaves];
+    __shared__ int s_halo_last[2];
     __shared__ int s_tile[4];
     __shared__ int64_t s_kexcl;
     __shared__ int s_abort;
@@ -2440,34 +2490,36 @@ bf_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
     // round k: stage and count group k+1 (slot (k+1)&1), then emit group k (slot k&1);
     // round -1 stages and counts group 0
     uint64_t pv = 0;   // thread 0: the emitted group's prefix word, loaded before the count
-    uint64_t m_cur = 0, m_next = 0;   // candidate masks of the emitted and the counted group
+    BfPre pre_cur{}, pre_next{};      // what the count phases handed the emitted and the counted group
     for (int64_t k = -1; k < R; ++k) {
         const int64_t gc = (k + 1) * W + w;            // the group counted this round
         if (tid == 0 && k >= 0)
             pv = ld_agent(a.pref + k * W + w);
         if (k + 1 < R) {
-            uint8_t *const slot = s_in + ((k + 1) & 1) * kIn;
-            __syncthreads();                           // the previous round's emit is done with slot
+            const int sl = (int)((k + 1) & 1);
+            uint8_t *const slot = s_in + sl * kIn;
+            __syncthreads();                           // the previous round's emit is done with slot sl
             bf_stage_to_lds(stage, tid, slot);
+            if (tid == 0)
+                s_halo_last[sl] = INT_MIN;
             const int64_t gn = k + 2 < R ? gc + W : gc;
             bf_prefetch<false>(p, gn, tid, stage);
             __syncthreads();
-            const BfLds lc{slot, s_out, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
-            m_next = bf_mask_count_staged(p, gc, lc, a, tag);
+            pre_next = bf_pre_count_staged(p, gc, slot, s_pat, a, tag, s_wsum[sl], s_wmax[sl], &s_halo_last[sl]);
         }
         if (k < 0) {
-            m_cur = m_next;
+            pre_cur = pre_next;
             continue;
         }
         const int64_t g = k * W + w;
-        if (tid == 0)
-            s_halo_last = INT_MIN;
-        __syncthreads();                               // the count is done with s_wsum
-        const BfLds le{s_in + (k & 1) * kIn, s_out, s_pat, s_rep, s_wsum, s_wmax, &s_halo_last, s_tile};
+        const int se = (int)(k & 1);
+        if (k + 1 >= R)
+            __syncthreads();                           // no count phase this round: order the last emit first
+        const BfLds le{s_in + se * kIn, s_out, s_pat, s_rep, s_wsum[se], s_wmax[se], &s_halo_last[se], s_tile};
         LateK lk{&a, tag, g, pv, &s_kexcl, &s_abort};
-        if (!bf_emit_staged<FULL4, true, kKeepOutCap>(p, g, lk, le, rep_v, rp, &m_cur))
+        if (!bf_emit_staged<FULL4, true, kKeepOutCap>(p, g, lk, le, rep_v, rp, &pre_cur))
             return;
-        m_cur = m_next;
+        pre_cur = pre_next;
     }
 }
 
