@@ -330,6 +330,70 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
     return v;
 }
 
+// Wave64 inclusive scans with DPP: row shifts by 1, 2, 4 and 8 within each row of 16
+// lanes, then the last lane of a row broadcast to the rows after it (row_bcast15 to
+// rows 1 and 3, row_bcast31 to rows 2 and 3).  A lane whose DPP source is disabled or
+// out of its row receives the operation's identity.
+__device__ __forceinline__ uint32_t wave_incl_add_u32(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+__device__ __forceinline__ int wave_incl_max_i32(int x)
+{
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x118, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x142, 0xa, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+__device__ __forceinline__ int wave_shr1_i32(int x, int first)
+{
+    return __builtin_amdgcn_update_dpp(first, x, 0x138, 0xf, 0xf, false);
+}
+
+// The same scan over 32-bit carry tables: lane i ends with lane0 ∘ ... ∘ lanei.
+// The identity table (carry c -> c, nothing kept) fills the lanes without a source.
+template <int LT, int CTRL, int ROWS>
+__device__ __forceinline__ Table<LT> dpp_compose_step(const Table<LT> &t)
+{
+    Table<LT> o;
+#pragma unroll
+    for (int c = 0; c < LT; ++c)
+        o.e[c] = (uint32_t)__builtin_amdgcn_update_dpp((int)((uint32_t)c << 24), (int)t.e[c], CTRL, ROWS, 0xf, false);
+    return tcompose<LT>(o, t);
+}
+
+template <int LT>
+__device__ __forceinline__ Table<LT> wave_incl_table(Table<LT> t)
+{
+    t = dpp_compose_step<LT, 0x111, 0xf>(t);
+    t = dpp_compose_step<LT, 0x112, 0xf>(t);
+    t = dpp_compose_step<LT, 0x114, 0xf>(t);
+    t = dpp_compose_step<LT, 0x118, 0xf>(t);
+    t = dpp_compose_step<LT, 0x142, 0xa>(t);
+    t = dpp_compose_step<LT, 0x143, 0xc>(t);
+    return t;
+}
+
+// The table of the lane before (wave_shr1), the identity on lane 0.
+template <int LT>
+__device__ __forceinline__ Table<LT> wave_shr1_table(const Table<LT> &t)
+{
+    Table<LT> o;
+#pragma unroll
+    for (int c = 0; c < LT; ++c)
+        o.e[c] = (uint32_t)__builtin_amdgcn_update_dpp((int)((uint32_t)c << 24), (int)t.e[c], 0x138, 0xf, 0xf, false);
+    return o;
+}
+
 // ---- group staging ----------------------------------------------------------
 // LDS image of a group of GT tiles: [kHalo bytes before | GT tiles | kHalo bytes after].
 template <int GT>
@@ -476,8 +540,8 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
                 cnt += __builtin_popcount(m);
             }
         }
-        cnt = wave_sum_u32(cnt);
-        if (lane == 0)
+        cnt = wave_incl_add_u32(cnt);
+        if (lane == kWave - 1)
             s_red[wave] = cnt;
         __syncthreads();
         if (tid == 0)
@@ -501,8 +565,8 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
 #pragma unroll
         for (int k = 1; k < GT; ++k)
             mine = tcompose<LT>(mine, segment_table<LT>(seg_mask(b0 + k * kBytesPerThread), p.L));
-        const Table<LT> t = wave_reduce_table<LT>(mine, lane);
-        if (lane == 0) {
+        const Table<LT> t = wave_incl_table<LT>(mine);   // lane 63: the wave's composition
+        if (lane == kWave - 1) {
 #pragma unroll
             for (int c = 0; c < LT; ++c)
                 s_wt[wave][c] = t.e[c];
@@ -1125,19 +1189,9 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
         if constexpr (MODE == kModeBorderFree || MODE == kModeBitmap) {
             const int kc = __builtin_popcount(m);
             const int lastq = m ? (seg + 31 - __builtin_clz(m)) : kNegInf;
-            int ks = kc, lq = lastq;
-#pragma unroll
-            for (int dd = 1; dd < kWave; dd <<= 1) {
-                const int uk = __shfl_up(ks, dd, kWave);
-                const int ul = __shfl_up(lq, dd, kWave);
-                if (lane >= dd) {
-                    ks += uk;
-                    lq = ul > lq ? ul : lq;
-                }
-            }
-            int lq_ex = __shfl_up(lq, 1, kWave);
-            if (lane == 0)
-                lq_ex = kNegInf;
+            const int ks = (int)wave_incl_add_u32((uint32_t)kc);
+            const int lq = wave_incl_max_i32(lastq);
+            const int lq_ex = wave_shr1_i32(lq, kNegInf);
             if (lane == kWave - 1) {
                 s_wsum[wave] = ks;
                 s_wmax[wave] = lq;
@@ -1167,22 +1221,8 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
         } else {
             // carry tables: wave-inclusive scan, then across the four waves
             const Table<LT> t = segment_table<LT>(m, L);
-            Table<LT> inc = t;
-#pragma unroll
-            for (int dd = 1; dd < kWave; dd <<= 1) {
-                Table<LT> nb;
-#pragma unroll
-                for (int c = 0; c < LT; ++c)
-                    nb.e[c] = __shfl_up(inc.e[c], dd, kWave);
-                if (lane >= dd)
-                    inc = tcompose<LT>(nb, inc);
-            }
-            Table<LT> ex;
-#pragma unroll
-            for (int c = 0; c < LT; ++c) {
-                const uint32_t v = __shfl_up(inc.e[c], 1, kWave);
-                ex.e[c] = lane == 0 ? ((uint32_t)c << 24) : v;
-            }
+            const Table<LT> inc = wave_incl_table<LT>(t);
+            const Table<LT> ex = wave_shr1_table<LT>(inc);
             if (lane == kWave - 1) {
 #pragma unroll
                 for (int c = 0; c < LT; ++c)
@@ -1207,13 +1247,8 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
             int lst;
             kept = LT == 2 ? greedy32_pair(m, c_i, &lst) : greedy32(m, c_i, L, &lst);
             const int lastq = kept ? seg + lst : kNegInf;
-            int ks = __builtin_popcount(kept), lq = lastq;
-#pragma unroll
-            for (int dd = kWave / 2; dd > 0; dd >>= 1) {
-                ks += __shfl_xor(ks, dd, kWave);
-                const int o = __shfl_xor(lq, dd, kWave);
-                lq = o > lq ? o : lq;
-            }
+            const int ks = __builtin_amdgcn_readlane((int)wave_incl_add_u32((uint32_t)__builtin_popcount(kept)), kWave - 1);
+            const int lq = __builtin_amdgcn_readlane(wave_incl_max_i32(lastq), kWave - 1);
             if (lane == 0) {
                 s_wsum[wave] = ks;
                 s_wmax[wave] = lq;
@@ -1364,33 +1399,9 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // wave64 inclusive scans with DPP (row_shr 1/2/4/8 inside rows of 16 lanes, then
 // row_bcast 15 and 31 across rows).  Lanes without a source read the identity.
-__device__ __forceinline__ uint32_t wave_incl_add_u32(uint32_t x)
-{
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
-    return x;
-}
 
-__device__ __forceinline__ int wave_incl_max_i32(int x)
-{
-    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x111, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x112, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x114, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x118, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x142, 0xa, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x143, 0xc, 0xf, false));
-    return x;
-}
 
 // previous lane's value (wave_shr 1); lane 0 reads `first`
-__device__ __forceinline__ int wave_shr1_i32(int x, int first)
-{
-    return __builtin_amdgcn_update_dpp(first, x, 0x138, 0xf, 0xf, false);
-}
 
 __device__ __forceinline__ uint64_t lds_load8(const uint8_t *p)
 {
