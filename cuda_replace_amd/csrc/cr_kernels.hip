@@ -1247,16 +1247,16 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
             int lst;
             kept = LT == 2 ? greedy32_pair(m, c_i, &lst) : greedy32(m, c_i, L, &lst);
             const int lastq = kept ? seg + lst : kNegInf;
-            const int ks = __builtin_amdgcn_readlane((int)wave_incl_add_u32((uint32_t)__builtin_popcount(kept)), kWave - 1);
-            const int lq = __builtin_amdgcn_readlane(wave_incl_max_i32(lastq), kWave - 1);
-            if (lane == 0) {
-                s_wsum[wave] = ks;
-                s_wmax[wave] = lq;
-            }
-            __syncthreads();
-            for (int w = 0; w < kWaves; ++w) {
-                tile_k += s_wsum[w];
-                tile_last = s_wmax[w] > tile_last ? s_wmax[w] : tile_last;
+            // kept starts in the tile from its composed table; the last one only for
+            // the tile's record
+            tile_k = (int)(tsel<LT>(agg, (uint32_t)c_t) & 0xffffffu);
+            if (p.info) {
+                const int lq = __builtin_amdgcn_readlane(wave_incl_max_i32(lastq), kWave - 1);
+                if (lane == 0)
+                    s_wmax[wave] = lq;
+                __syncthreads();
+                for (int w = 0; w < kWaves; ++w)
+                    tile_last = s_wmax[w] > tile_last ? s_wmax[w] : tile_last;
             }
         }
 

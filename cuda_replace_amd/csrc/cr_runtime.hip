@@ -659,7 +659,10 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
     p.chunks = s->d_chunks;
     p.block_tabs = s->d_blocks;
     p.chunks_pre = s->d_pre;
-    p.info = s->d_info;
+    // per-tile records serve later prefix queries and the last kept start; a device
+    // pass that asks for neither drops them (bitmap mode always reads its own)
+    const bool records = fixed_cap < 0 || want_last || mode == kModeBitmap;
+    p.info = records ? s->d_info : nullptr;
     p.bitmap = s->d_bitmap;
     // the 32-byte-per-thread count hands its candidate masks to the emit
     const bool have_mask_buf = s->bitmap_words >= (size_t)num_tiles * (kTile / 32);
