@@ -2534,6 +2534,340 @@ bf_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
     }
 }
 
+// ---- held-group single launch for 2-state carry tables -----------------------
+// A 2-byte pattern of one repeated byte (aa, "  ", ...) runs with 2-state carry
+// tables.  One persistent launch does the whole pass as bf_sweep_keep_kernel does for
+// border-free patterns: worker w stages and counts group (k+1)W + w into one LDS slot
+// and emits group kW + w from the other, so the source is read once.  Its count phase
+// builds each thread's table over its 64 bytes (two 32-position segments), scans
+// the tables across the wave with DPP and publishes the group's table packed into
+// one tagged word; the scanner composes the tables in group order into each group's
+// (kept starts before it, carry into it).  The emit needs that carry before it can
+// choose its kept starts, so it waits first, then resolves each thread's carry from
+// the wave scans kept from its count phase.
+constexpr int kTblOutCap = kBfGroup + 2048;
+
+// group table entry c: carry_out << 24 | kept (< 2^15) -> 16 packed bits
+__device__ __forceinline__ uint64_t tbl2_pack(const Table<2> &t)
+{
+    auto e = [](uint32_t v) { return ((v >> 24) << 15) | (v & 0x7fffu); };
+    return (uint64_t)(e(t.e[0]) | (e(t.e[1]) << 16));
+}
+__device__ __forceinline__ Table<2> tbl2_unpack(uint64_t x)
+{
+    Table<2> t;
+    t.e[0] = (((uint32_t)(x >> 15) & 1u) << 24) | ((uint32_t)x & 0x7fffu);
+    t.e[1] = (((uint32_t)(x >> 31) & 1u) << 24) | ((uint32_t)(x >> 16) & 0x7fffu);
+    return t;
+}
+constexpr int kTblCarryShift = 40;   // prefix word: tag | carry << 40 | kept before
+
+__device__ __forceinline__ void tbl_scanner(const ReplaceParams &p, const SweepArgs &a, int lane, uint64_t tag,
+                                            int64_t G)
+{
+    constexpr int KB = 16;
+    uint64_t k_run = 0;
+    uint32_t c_run = 0;
+    const int64_t nb = (G + kWave - 1) / kWave;
+    int spins = 0;
+    for (int64_t b = 0; b < nb;) {
+        uint64_t v[KB];
+#pragma unroll
+        for (int i = 0; i < KB; ++i) {
+            const int64_t idx = (b + i) * kWave + lane;
+            v[i] = ld_agent(a.agg + (idx < G ? idx : G - 1));
+        }
+        int done = 0;
+        bool go = true;
+#pragma unroll
+        for (int i = 0; i < KB; ++i) {
+            const int64_t idx = (b + i) * kWave + lane;
+            const bool ok = idx >= G || (v[i] & kTagMask) == tag;
+            go = go && b + i < nb && __builtin_amdgcn_ballot_w64(!ok) == 0;
+            if (go) {
+                const Table<2> t = idx < G ? tbl2_unpack(v[i]) : tidentity<2>();
+                const Table<2> inc = wave_incl_table<2>(t);
+                const Table<2> ex = wave_shr1_table<2>(inc);
+                const uint32_t e = tsel<2>(ex, c_run);
+                if (idx < G)
+                    st_agent(a.pref + idx, tag | ((uint64_t)(e >> 24) << kTblCarryShift) | (k_run + (e & 0xffffffu)));
+                const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)inc.e[0], kWave - 1);
+                const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)inc.e[1], kWave - 1);
+                const uint32_t tc = c_run ? t1 : t0;
+                k_run += tc & 0xffffffu;
+                c_run = tc >> 24;
+                done = i + 1;
+            }
+        }
+        b += done;
+        if (done == 0) {
+            if (sweep_should_give_up(a, tag, ++spins)) {
+                if (lane == 0)
+                    st_agent(a.flag, tag | 1);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        } else {
+            spins = 0;
+        }
+    }
+    if (lane == 0)
+        st_agent(&p.stats->k_total, tag | k_run);
+}
+
+// What a group's count phase hands its emit: the thread's two segment masks and its
+// wave-exclusive table (the waves' inclusive tables are in LDS).
+struct TblPre {
+    uint32_t m0, m1;
+    Table<2> ex;
+};
+
+__global__ void __launch_bounds__(kThreads)
+tbl_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
+{
+    constexpr int kIn = GroupShape<kGroupTiles>::kIn;
+    __shared__ __align__(16) uint8_t s_in[2 * kIn];
+    __shared__ __align__(16) uint8_t s_out[kTblOutCap];
+    __shared__ uint8_t s_pat[kMaxSinglePassPat];
+    __shared__ __align__(16) uint8_t s_rep[kRepLds + 64];
+    __shared__ uint32_t s_wtab[2][kWaves][2];   // per slot: each wave's inclusive table
+    __shared__ int s_wmax[kWaves];
+    __shared__ int s_tile[4];
+    __shared__ int64_t s_kexcl;
+    __shared__ int s_carry;
+    __shared__ int s_abort;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    for (int i = tid; i < p.L; i += kThreads)
+        s_pat[i] = p.pat[i];
+    for (int i = tid; i < p.r && i < kRepLds; i += kThreads)
+        s_rep[i] = p.rep[i];
+    const uint64_t tag = (uint64_t)p.epoch << kEpochShift;
+    const int64_t G = p.num_groups;
+    if (tid == 0 && blockIdx.x == gridDim.x - 1)
+        st_agent(a.arrive, tag | 1);
+    if (blockIdx.x == 0) {
+        if (tid < kWave)
+            tbl_scanner(p, a, lane, tag, G);
+        return;
+    }
+    const int64_t W = gridDim.x - 1;
+    const int64_t w = blockIdx.x - 1;
+    const int64_t R = w < G ? (G - w + W - 1) / W : 0;
+    if (tid == 0)
+        s_abort = 0;
+    __syncthreads();
+    const uint4 rep_v = lds_load16(s_rep);
+    const BfRep rp = bf_rep_parts(rep_v, p.r);
+    if (R == 0)
+        return;
+    const int L = p.L;
+    const int r = p.r;
+    const int64_t n = p.n;
+    const int64_t diff = (int64_t)r - L;
+    const int b = tid * kBfSeg;   // my 64 bytes of a group
+    BfStage stage;
+    bf_prefetch<false>(p, w, tid, stage);
+    TblPre cur{}, nxt{};
+    for (int64_t k = -1; k < R; ++k) {
+        const int64_t gc = (k + 1) * W + w;
+        uint64_t pv = 0;
+        if (tid == 0 && k >= 0)
+            pv = ld_agent(a.pref + k * W + w);
+        if (k + 1 < R) {
+            // count phase of group gc in slot sl
+            const int sl = (int)((k + 1) & 1);
+            uint8_t *const slot = s_in + sl * kIn;
+            __syncthreads();                           // the previous round's emit is done with slot sl
+            bf_stage_to_lds(stage, tid, slot);
+            const int64_t gn = k + 2 < R ? gc + W : gc;
+            bf_prefetch<false>(p, gn, tid, stage);
+            __syncthreads();
+            const int64_t gb = gc * (int64_t)kBfGroup + b;
+            nxt.m0 = cand_mask(p, slot + kHalo, s_pat, b, gb);
+            nxt.m1 = cand_mask(p, slot + kHalo, s_pat, b + 32, gb + 32);
+            const Table<2> t = tcompose<2>(segment_table<2>(nxt.m0, L), segment_table<2>(nxt.m1, L));
+            const Table<2> inc = wave_incl_table<2>(t);
+            nxt.ex = wave_shr1_table<2>(inc);
+            if (lane == kWave - 1) {
+                s_wtab[sl][wave][0] = inc.e[0];
+                s_wtab[sl][wave][1] = inc.e[1];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                Table<2> grp = tidentity<2>();
+#pragma unroll
+                for (int v = 0; v < kWaves; ++v) {
+                    Table<2> wt;
+                    wt.e[0] = s_wtab[sl][v][0];
+                    wt.e[1] = s_wtab[sl][v][1];
+                    grp = tcompose<2>(grp, wt);
+                }
+                st_agent(a.agg + gc, tag | tbl2_pack(grp));
+            }
+        }
+        if (k < 0) {
+            cur = nxt;
+            continue;
+        }
+        // emit phase of group g from slot se: wait for (kept before, carry in)
+        const int64_t g = k * W + w;
+        const int se = (int)(k & 1);
+        if (tid == 0) {
+            uint64_t v = pv;
+            int spins = 0;
+            while ((v & kTagMask) != tag) {
+                if (sweep_should_give_up(a, tag, ++spins)) {
+                    st_agent(a.flag, tag | 1);
+                    s_abort = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                v = ld_agent(a.pref + g);
+            }
+            s_kexcl = (int64_t)(v & ((1ull << kTblCarryShift) - 1));
+            s_carry = (int)((v >> kTblCarryShift) & 0xffu);
+        }
+        __syncthreads();
+        if (s_abort)
+            return;
+        const int64_t k_before = s_kexcl;
+        const int c_in = s_carry;
+        const int64_t t0 = g * kGroupTiles;
+        const int64_t G0 = g * (int64_t)kBfGroup;
+        const int valid = (int)((n - G0) < kBfGroup ? (n - G0) : kBfGroup);
+        const bool last_group = G0 + kBfGroup >= n;
+        Table<2> wpre = tidentity<2>(), agg = tidentity<2>();
+#pragma unroll
+        for (int v = 0; v < kWaves; ++v) {
+            Table<2> wt;
+            wt.e[0] = s_wtab[se][v][0];
+            wt.e[1] = s_wtab[se][v][1];
+            if (v < wave)
+                wpre = tcompose<2>(wpre, wt);
+            agg = tcompose<2>(agg, wt);
+        }
+        const uint32_t me = tsel<2>(tcompose<2>(wpre, cur.ex), (uint32_t)c_in);
+        const int c_t = (int)(me >> 24);              // leading bytes of my 64 covered
+        const int k_loc = (int)(me & 0xffffffu);      // kept starts of the group before my bytes
+        int lst0, lst1;
+        const uint32_t kept0 = greedy32_pair(cur.m0, c_t, &lst0);
+        int c_mid = lst0 + L - 32;
+        c_mid = c_mid < 0 ? 0 : c_mid;
+        const uint32_t kept1 = greedy32_pair(cur.m1, c_mid, &lst1);
+        const uint32_t ge = tsel<2>(agg, (uint32_t)c_in);
+        const int c_next = last_group ? 0 : (int)(ge >> 24);
+        const int64_t K = (int64_t)(ge & 0xffffffu);
+        const int64_t out_len = (int64_t)valid - c_in + c_next + K * diff;
+        const int64_t out_base = G0 + c_in + k_before * diff;
+        const int jend = (valid - b) < kBfSeg ? ((valid - b) > 0 ? valid - b : 0) : kBfSeg;
+        int j = c_t < jend ? c_t : jend;
+        const int64_t o = (int64_t)(b + j - c_in) + (int64_t)k_loc * diff;   // my first output byte
+        uint64_t M = ((uint64_t)kept1 << 32) | kept0;
+        M = j >= 64 ? 0ull : (M & (~0ull << j));
+
+        if (p.info) {
+            // tile records: tile 0 is threads 0-127, tile 1 threads 128-255
+            const int lastq = kept1 ? b + 32 + lst1 : kept0 ? b + lst0 : kNegInf;
+            const int lq = __builtin_amdgcn_readlane(wave_incl_max_i32(lastq), kWave - 1);
+            if (lane == 0)
+                s_wmax[wave] = lq;
+            if (tid == 128) {
+                s_tile[0] = k_loc;
+                s_tile[1] = c_t;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                const int l0 = max(s_wmax[0], s_wmax[1]), l1 = max(s_wmax[2], s_wmax[3]);
+                TileInfo ra;
+                ra.k_excl = k_before;
+                ra.last_abs = l0 >= 0 ? G0 + l0 : -1;
+                ra.carry_in = c_in;
+                ra.mode = kModeTable;
+                p.info[t0] = ra;
+                if (t0 + 1 < p.num_tiles) {
+                    TileInfo rb;
+                    rb.k_excl = k_before + s_tile[0];
+                    rb.last_abs = l1 >= 0 ? G0 + l1 : -1;
+                    rb.carry_in = s_tile[1];
+                    rb.mode = kModeTable;
+                    p.info[t0 + 1] = rb;
+                }
+            }
+        }
+        const uint8_t *const s_src = s_in + se * kIn + kHalo + b;
+        if (!p.write_output) {
+            // counts and records only
+        } else if (out_len + 16 <= kTblOutCap && r <= kRepLds) {
+            int oo = (int)o;
+            for (;;) {
+                const int q = M ? (int)__builtin_ctzll(M) : jend;
+                bf_copy_run(s_out + oo, s_src + j, q - j);
+                oo += q - j;
+                if (q >= jend)
+                    break;
+                if (r == 1) {
+                    // this kept start and the ones every L bytes after it: one run of
+                    // replacement bytes
+                    int c = 1;
+                    while (q + c * L < 64 && ((M >> (q + c * L)) & 1ull))
+                        ++c;
+                    const uint32_t b4 = (rep_v.x & 0xffu) * 0x01010101u;
+                    const uint4 fill = {b4, b4, b4, b4};
+                    int f = 0;
+                    for (; f + 16 <= c; f += 16)
+                        lds_store16(s_out + oo + f, fill);
+                    if (f < c)
+                        store_tail(s_out + oo + f, fill, c - f);
+                    oo += c;
+                    j = q + c * L;
+                } else {
+                    bf_put_rep_lds(s_out + oo, rp, rep_v, s_rep, r);
+                    oo += r;
+                    j = q + L;
+                }
+                M = j >= 64 ? 0ull : (M & (~0ull << j));
+                if (j >= jend)
+                    break;
+            }
+            __syncthreads();
+            uint8_t *const out_first = p.dst + out_base;
+            const int len = (int)out_len;
+            for (int c = tid * 16; c < len; c += kThreads * 16) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(s_out + c);
+                if (c + 16 <= len)
+                    __builtin_memcpy(out_first + c, &v, 16);
+                else
+                    store_tail(out_first + c, v, len - c);
+            }
+        } else {
+            // output larger than the image (or a long replacement): every thread stores
+            // its pieces straight to HBM
+            uint8_t *out = p.dst + out_base + o;
+            for (;;) {
+                const int q = M ? (int)__builtin_ctzll(M) : jend;
+                if (q > j)
+                    store_from_lds(out, s_src, j, q - j);
+                out += q - j;
+                if (q >= jend)
+                    break;
+                if (r <= kRepLds)
+                    store_from_lds(out, s_rep, 0, r);
+                else
+                    store_from_global(out, p.rep, r);
+                out += r;
+                j = q + L;
+                M = j >= 64 ? 0ull : (M & (~0ull << j));
+                if (j >= jend)
+                    break;
+            }
+            __syncthreads();   // the pieces were read from the slot: done before it is restaged
+        }
+        cur = nxt;
+    }
+}
+
 // ---- batches of independent documents ---------------------------------------
 // The count and emit launches of a batch round run the same per-group code as a
 // single buffer, on a per-document view built from the round's BatchDoc.
@@ -4184,6 +4518,15 @@ hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, i
     if (p.L >= 4)
         return img ? sweep_t<true, true>(p, a, s) : sweep_t<true, false>(p, a, s);
     return img ? sweep_t<false, true>(p, a, s) : sweep_t<false, false>(p, a, s);
+}
+
+hipError_t launch_tbl_sweep(const ReplaceParams &p, uint64_t *sweep, int spin_limit, hipStream_t s)
+{
+    const int64_t G = p.num_groups;
+    SweepArgs a{sweep, sweep + G, sweep + 2 * G, sweep + 2 * G + 1, 1, spin_limit};
+    const int64_t grid = std::min(G + 1, resident_grid(tbl_sweep_keep_kernel));
+    hipLaunchKernelGGL(tbl_sweep_keep_kernel, dim3((unsigned)std::max<int64_t>(grid, 2)), dim3(kThreads), 0, s, p, a);
+    return hipGetLastError();
 }
 
 int64_t sweep_words(int64_t groups) { return 2 * groups + 2; }

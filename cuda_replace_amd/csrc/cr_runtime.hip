@@ -60,6 +60,8 @@ constexpr bool single_launch(int v) { return v == kBfSweep || v == kBfKeep; }
 struct TuneEntry {
     int gt = 0;
     float ms[3] = {0, 0, 0};   // per candidate, best of three
+    int single = -1;           // 2-state tables: 1 the single launch, 0 the three launches, -1 not timed
+    float single_ms = -1.f;    // ... the single launch's best time (1e30: it gave up)
     int variant = -1;          // border-free class: the chosen BfVariant
     float vms[kBfVariants] = {0, 0, 0, 0, 0, 0};
 };
@@ -370,6 +372,38 @@ int64_t sweep_min_bytes()
         return INT64_MAX;
     const char *e = getenv("CR_SWEEP_MIN");
     return e ? (int64_t)atoll(e) : (64ll << 20);
+}
+
+// 2-state carry-table passes of this many bytes or more run as one persistent launch
+// (CR_TABLE_SWEEP=0 keeps them on the three launches, CR_TABLE_SWEEP_MIN moves the
+// threshold; read per pass)
+int64_t table_sweep_min_bytes()
+{
+    const char *off = getenv("CR_TABLE_SWEEP");
+    if (off && atoi(off) == 0)
+        return INT64_MAX;
+    const char *e = getenv("CR_TABLE_SWEEP_MIN");
+    return e ? (int64_t)atoll(e) : (64ll << 20);
+}
+
+// Whether a 2-state carry-table pass of n bytes runs as the single launch: never
+// with CR_TABLE_SWEEP=0, by the threshold alone when CR_TABLE_SWEEP_MIN is set, else
+// the tuned choice for the size bucket, else from 64 MiB.
+bool table_single_chosen(int device, int64_t n)
+{
+    const int64_t smin = table_sweep_min_bytes();
+    if (smin == INT64_MAX)
+        return false;
+    if (getenv("CR_TABLE_SWEEP_MIN"))
+        return n >= smin;
+    {
+        Tuner &t = tuner();
+        std::lock_guard<std::mutex> lk(t.mtx);
+        auto it = t.best.find(std::make_tuple(device, tune_class(kModeTable, 2), size_bucket(n)));
+        if (it != t.best.end() && it->second.single >= 0)
+            return it->second.single == 1;
+    }
+    return n >= smin;
 }
 
 int sweep_ahead()
@@ -770,15 +804,24 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             return rc;
         }
         p.dst = dst;
-        // border-free passes of a single chunk: one persistent launch when large enough
+        // border-free passes of a single chunk, and 2-state carry-table passes: one
+        // persistent launch when large enough
         bool swept = false;
-        if (!counted && single_launch(bf_var) && num_chunks == 1 && ensure_sweep(s, num_groups) == hipSuccess) {
+        const bool tbl_single = !counted && mode == kModeTable && table_size == 2 && num_chunks == 1 &&
+                                table_single_chosen(s->device, n);
+        const int64_t sweep_groups = tbl_single ? groups_of(num_tiles, kGroupTiles) : num_groups;
+        if ((tbl_single || (!counted && single_launch(bf_var) && num_chunks == 1)) &&
+            ensure_sweep(s, sweep_groups) == hipSuccess) {
+            ReplaceParams q = p;
+            q.num_groups = sweep_groups;
             if (kernel_ms)
                 CK(hipEventRecord(s->ev0, s->stream));
-            CK(launch_bf_sweep(p, s->d_sweep, sweep_ahead(), sweep_spin_limit(), bf_var == kBfKeep, s->stream));
+            CK(tbl_single ? launch_tbl_sweep(q, s->d_sweep, sweep_spin_limit(), s->stream)
+                          : launch_bf_sweep(q, s->d_sweep, sweep_ahead(), sweep_spin_limit(), bf_var == kBfKeep,
+                                            s->stream));
             if (kernel_ms)
                 CK(hipEventRecord(s->ev1, s->stream));
-            CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * num_groups, 8, hipMemcpyDeviceToHost, s->stream));
+            CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * sweep_groups, 8, hipMemcpyDeviceToHost, s->stream));
             if (want_last)
                 CK(launch_last_kept(p, s->d_query, s->stream));
             rc = read_stats(&kept, nullptr);   // synchronizes
@@ -885,7 +928,26 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
             // time per candidate; another size replaces the default only when it
             // is more than 3% faster.
             TuneEntry te;
+            // 2-state tables: the single launch is a candidate too
+            const int64_t g16 = groups_of(num_tiles, kGroupTiles);
+            const bool time_single = mode == kModeTable && table_size == 2 && !counted &&
+                                     table_sweep_min_bytes() != INT64_MAX && ensure_sweep(s, g16) == hipSuccess;
             for (int sweep = 0; sweep < 3; ++sweep) {
+                if (time_single) {
+                    ReplaceParams q = p;
+                    q.num_groups = g16;
+                    CK(next_epoch(s));
+                    q.epoch = s->epoch;
+                    CK(hipEventRecord(s->ev0, s->stream));
+                    CK(launch_tbl_sweep(q, s->d_sweep, sweep_spin_limit(), s->stream));
+                    CK(hipEventRecord(s->ev1, s->stream));
+                    CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * g16, 8, hipMemcpyDeviceToHost, s->stream));
+                    CK(hipStreamSynchronize(s->stream));
+                    float ms = 1e30f;
+                    if (*(const uint64_t *)(s->h_misc + 96) != (((uint64_t)s->epoch << kEpochShift) | 1u))
+                        CK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+                    te.single_ms = sweep == 0 || ms < te.single_ms ? ms : te.single_ms;
+                }
                 for (int ci = 0; ci < 3; ++ci) {
                     const int cand = kTuneCands[ci];
                     const int64_t ng = groups_of(num_tiles, cand);
@@ -907,6 +969,12 @@ int run_pass(Session *s, const uint8_t *d_src, int64_t n, const uint8_t *pat, in
                     te.gt = kTuneCands[ci];
                     best = te.ms[ci];
                 }
+            }
+            if (time_single) {
+                // the default (the single launch from the threshold on, else the three
+                // launches) stays unless the other is more than 3% faster
+                const bool dflt = n >= table_sweep_min_bytes();
+                te.single = dflt ? !(best < 0.97f * te.single_ms) : te.single_ms < 0.97f * best;
             }
             record_tuning(s->device, mode, table_size, n, te);
         }
@@ -1950,6 +2018,15 @@ int cr_autotune_report(char *buf, size_t cap)
                          first ? "" : ", ", std::get<0>(kv.first), kClassName[std::get<1>(kv.first)],
                          kBucket[std::get<2>(kv.first)], kVariant[te.variant], f(te.vms[0]), f(te.vms[1]),
                          f(te.vms[2]), f(te.vms[3]), f(te.vms[4]), f(te.vms[5]));
+            } else if (te.single >= 0) {
+                const double sm = te.single_ms >= 1e29f ? -1.0 : (double)te.single_ms;   // -1: gave up
+                snprintf(line, sizeof line,
+                         "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"form\": \"%s\", "
+                         "\"group_tiles\": %d, \"ms\": {\"1\": %.4f, \"2\": %.4f, \"4\": %.4f, "
+                         "\"single_launch_lds\": %.4f}}",
+                         first ? "" : ", ", std::get<0>(kv.first), kClassName[std::get<1>(kv.first)],
+                         kBucket[std::get<2>(kv.first)], te.single ? "single_launch_lds" : "three_launch", te.gt,
+                         te.ms[0], te.ms[1], te.ms[2], sm);
             } else {
                 snprintf(line, sizeof line,
                          "%s{\"device\": %d, \"class\": \"%s\", \"size\": \"%s\", \"group_tiles\": %d, "

@@ -80,11 +80,16 @@ def test_first_tuned_pass_measures_every_candidate(lib, pinned):
     assert t["size"] == "4-32MiB" and t["group_tiles"] in (1, 2, 4)
     ms = t["ms"]
     assert all(ms[k] > 0 for k in ("1", "2", "4"))
-    best = min(ms.values())
+    best = min(ms[k] for k in ("1", "2", "4"))
     if t["group_tiles"] == 2:   # the default stays unless another size is >3% faster
         assert best >= 0.97 * ms["2"] or ms["2"] == best
     else:
         assert ms[str(t["group_tiles"])] == best and best < 0.97 * ms["2"]
+    # 2-state tables also time their single launch; under 64 MiB the three launches
+    # stay unless the single launch is more than 3% faster (-1: it gave up)
+    sm = ms["single_launch_lds"]
+    assert t["form"] in ("single_launch_lds", "three_launch") and (sm > 0 or sm == -1)
+    assert (t["form"] == "single_launch_lds") == (sm > 0 and sm < 0.97 * best)
     b = classes["border_free"]
     forms = ("single_launch", "three_launch_64B", "tiles1_32B", "tiles2_32B", "tiles4_32B", "single_launch_lds")
     assert b["size"] == "4-32MiB" and b["form"] in forms
