@@ -2684,9 +2684,12 @@ tbl_sweep_keep_kernel(ReplaceParams p, SweepArgs a)
             const int64_t gn = k + 2 < R ? gc + W : gc;
             bf_prefetch<false>(p, gn, tid, stage);
             __syncthreads();
+            // the candidates of my 64 positions from the masked quad SADs (per-position
+            // compares at the input's edges)
             const int64_t gb = gc * (int64_t)kBfGroup + b;
-            nxt.m0 = cand_mask(p, slot + kHalo, s_pat, b, gb);
-            nxt.m1 = cand_mask(p, slot + kHalo, s_pat, b + 32, gb + 32);
+            const uint64_t m64 = gb < n ? bf_mask_seg<1>(p, slot + kHalo + b, s_pat, gb) : 0ull;
+            nxt.m0 = (uint32_t)m64;
+            nxt.m1 = (uint32_t)(m64 >> 32);
             const Table<2> t = tcompose<2>(segment_table<2>(nxt.m0, L), segment_table<2>(nxt.m1, L));
             const Table<2> inc = wave_incl_table<2>(t);
             nxt.ex = wave_shr1_table<2>(inc);
