@@ -3391,10 +3391,8 @@ __device__ __forceinline__ void fused_count_group(const ReplaceParams &p, const 
         if (p.masks)
             p.masks[ti * (kTile / 32) + tid] = occ;
     }
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1)
-        delta += __shfl_xor(delta, d, kWave);
-    if (lane == 0)
+    delta = (int)wave_incl_add_u32((uint32_t)delta);   // two's complement: signed sums
+    if (lane == kWave - 1)
         s_red[wave] = delta;
     __syncthreads();
     if (tid == 0)
@@ -3501,19 +3499,9 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
         }
         const int c_t = c_run;
         // in-tile scans: length change (sum) and coverage end (max)
-        int ds = d, le = e;
-#pragma unroll
-        for (int dd = 1; dd < kWave; dd <<= 1) {
-            const int ud = __shfl_up(ds, dd, kWave);
-            const int ue = __shfl_up(le, dd, kWave);
-            if (lane >= dd) {
-                ds += ud;
-                le = ue > le ? ue : le;
-            }
-        }
-        int le_ex = __shfl_up(le, 1, kWave);
-        if (lane == 0)
-            le_ex = kNegInf;
+        const int ds = (int)wave_incl_add_u32((uint32_t)d);   // two's complement: signed sums
+        const int le = wave_incl_max_i32(e);
+        const int le_ex = wave_shr1_i32(le, kNegInf);
         if (lane == kWave - 1) {
             s_wsum[wave] = ds;
             s_wmax[wave] = le;
