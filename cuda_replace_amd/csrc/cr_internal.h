@@ -298,6 +298,9 @@ hipError_t launch_bf_sweep(const ReplaceParams &p, uint64_t *sweep, int ahead, i
 // the held-group single launch of a 2-state carry-table pass (p.num_groups groups of
 // 16 KiB); same sweep words and give-up protocol as launch_bf_sweep
 hipError_t launch_tbl_sweep(const ReplaceParams &p, uint64_t *sweep, int spin_limit, hipStream_t s);
+// the held-group single launch of a fused multi-pattern pass (16 KiB groups)
+hipError_t launch_fused_sweep(const ReplaceParams &p, const FusedTable *t, uint64_t *sweep, int spin_limit,
+                              hipStream_t s);
 int64_t sweep_words(int64_t groups);
 hipError_t launch_bitmap_mark(const ReplaceParams &p, hipStream_t s);
 hipError_t launch_bitmap_greedy(const ReplaceParams &p, hipStream_t s);

@@ -3596,6 +3596,308 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
     }
 }
 
+// ---- held-group single launch of a fused multi-pattern pass -------------------
+// The structure of bf_sweep_keep_kernel for the fused table: worker w counts group
+// (k+1)W + w into one LDS slot and emits group kW + w from the other.  The count
+// phase probes every candidate of a thread's 64 bytes (the table's shared first
+// byte), keeps the occurrence mask, the first two entries, the signed length change
+// and the wave scans of length changes and coverage ends, and the halo probes give
+// the coverage of the group's first bytes; the scanner sums the groups' length
+// changes in order (signed, 48 bits in the tagged word).  All occurrences are kept and
+// disjoint, so the emit builds its LDS image before it waits for the group's prefix.
+constexpr int kFusOutCap = kBfGroup + 2048;
+constexpr uint64_t kMask48 = (1ull << kEpochShift) - 1;
+
+__device__ __forceinline__ int64_t sext48(uint64_t v)
+{
+    return (int64_t)(v << (64 - kEpochShift)) >> (64 - kEpochShift);
+}
+
+__device__ __forceinline__ void fused_scanner(const ReplaceParams &p, const SweepArgs &a, int lane, uint64_t tag,
+                                              int64_t G)
+{
+    constexpr int KB = 16;
+    int64_t running = 0;
+    const int64_t nb = (G + kWave - 1) / kWave;
+    int spins = 0;
+    for (int64_t b = 0; b < nb;) {
+        uint64_t v[KB];
+#pragma unroll
+        for (int i = 0; i < KB; ++i) {
+            const int64_t idx = (b + i) * kWave + lane;
+            v[i] = ld_agent(a.agg + (idx < G ? idx : G - 1));
+        }
+        int done = 0;
+        bool go = true;
+#pragma unroll
+        for (int i = 0; i < KB; ++i) {
+            const int64_t idx = (b + i) * kWave + lane;
+            const bool ok = idx >= G || (v[i] & kTagMask) == tag;
+            go = go && b + i < nb && __builtin_amdgcn_ballot_w64(!ok) == 0;
+            if (go) {
+                const int32_t c = idx < G ? (int32_t)(uint32_t)(v[i] & 0xffffffffu) : 0;
+                const int32_t inc = (int32_t)wave_incl_add_u32((uint32_t)c);   // block sums fit 32 bits
+                if (idx < G)
+                    st_agent(a.pref + idx, tag | ((uint64_t)(running + inc - c) & kMask48));
+                running += (int32_t)__builtin_amdgcn_readlane(inc, kWave - 1);
+                done = i + 1;
+            }
+        }
+        b += done;
+        if (done == 0) {
+            if (sweep_should_give_up(a, tag, ++spins)) {
+                if (lane == 0)
+                    st_agent(a.flag, tag | 1);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        } else {
+            spins = 0;
+        }
+    }
+    if (lane == 0)
+        st_agent(&p.stats->k_total, tag | ((uint64_t)running & kMask48));
+}
+
+// What a group's count phase hands its emit (per thread).
+struct FusPre {
+    uint64_t occ;        // occurrences starting in my 64 bytes
+    uint64_t e0, e1;     // entries of my first two occurrences
+    int d;               // my summed length change
+    int ds;              // wave-inclusive sum of d
+    int le;              // wave-inclusive max of my coverage end
+};
+
+__global__ void __launch_bounds__(kThreads)
+fused_sweep_keep_kernel(ReplaceParams p, const FusedTable *t, SweepArgs a)
+{
+    constexpr int kIn = GroupShape<kGroupTiles>::kIn;
+    __shared__ __align__(16) uint8_t s_in[2 * kIn];
+    __shared__ __align__(16) uint8_t s_out[kFusOutCap];
+    __shared__ int s_wsum[2][kWaves];
+    __shared__ int s_wmax[2][kWaves];
+    __shared__ int s_halo[2];
+    __shared__ int64_t s_kexcl;
+    __shared__ int s_abort;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const uint64_t tag = (uint64_t)p.epoch << kEpochShift;
+    const int64_t G = p.num_groups;
+    if (tid == 0 && blockIdx.x == gridDim.x - 1)
+        st_agent(a.arrive, tag | 1);
+    if (blockIdx.x == 0) {
+        if (tid < kWave)
+            fused_scanner(p, a, lane, tag, G);
+        return;
+    }
+    const int64_t W = gridDim.x - 1;
+    const int64_t w = blockIdx.x - 1;
+    const int64_t R = w < G ? (G - w + W - 1) / W : 0;
+    if (tid == 0)
+        s_abort = 0;
+    __syncthreads();
+    if (R == 0)
+        return;
+    const int64_t n = p.n;
+    const uint8_t first = (uint8_t)p.pat4;
+    const int b = tid * kBfSeg;   // my 64 bytes of a group
+    BfStage stage;
+    bf_prefetch<false>(p, w, tid, stage);
+    FusPre cur{}, nxt{};
+    for (int64_t k = -1; k < R; ++k) {
+        const int64_t gc = (k + 1) * W + w;
+        uint64_t pv = 0;
+        if (tid == 0 && k >= 0)
+            pv = ld_agent(a.pref + k * W + w);
+        if (k + 1 < R) {
+            // count phase of group gc in slot sl
+            const int sl = (int)((k + 1) & 1);
+            uint8_t *const slot = s_in + sl * kIn;
+            __syncthreads();                           // the previous round's emit is done with slot sl
+            bf_stage_to_lds(stage, tid, slot);
+            if (tid == 0)
+                s_halo[sl] = 0;
+            const int64_t gn = k + 2 < R ? gc + W : gc;
+            bf_prefetch<false>(p, gn, tid, stage);
+            __syncthreads();
+            const int64_t G0c = gc * (int64_t)kBfGroup;
+            const int64_t gb = G0c + b;
+            const uint8_t *sp = slot + kHalo + b;
+            uint64_t m = gb < n ? bf_cand64<false>(p, sp, slot, gb) : 0ull;   // the shared first byte
+            nxt.occ = 0;
+            nxt.e0 = nxt.e1 = 0;
+            int d = 0, e = kNegInf, nk = 0;
+            while (m) {
+                const int q = (int)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint64_t en = fused_probe(t, sp + q, n - (gb + q));
+                if (en) {
+                    nxt.occ |= 1ull << q;
+                    d += fused_entry_rep_len(en) - fused_entry_pat_len(en);
+                    e = b + q + fused_entry_pat_len(en);
+                    nxt.e1 = nk == 1 ? en : nxt.e1;
+                    nxt.e0 = nk == 0 ? en : nxt.e0;
+                    ++nk;
+                }
+            }
+            // an occurrence starting in the max_len-1 bytes left of the group covers
+            // the group's first bytes
+            if (G0c > 0 && tid < t->max_len - 1) {
+                const int q = -1 - tid;
+                if (slot[kHalo + q] == first) {
+                    const uint64_t en = fused_probe(t, slot + kHalo + q, n - (G0c + q));
+                    if (en && q + fused_entry_pat_len(en) > 0)
+                        atomicMax(&s_halo[sl], q + fused_entry_pat_len(en));
+                }
+            }
+            nxt.d = d;
+            nxt.ds = (int)wave_incl_add_u32((uint32_t)d);   // two's complement: signed sums
+            nxt.le = wave_incl_max_i32(e);
+            if (lane == kWave - 1) {
+                s_wsum[sl][wave] = nxt.ds;
+                s_wmax[sl][wave] = nxt.le;
+            }
+            __syncthreads();
+            if (tid == 0)
+                st_agent(a.agg + gc,
+                         tag | (uint64_t)(uint32_t)(s_wsum[sl][0] + s_wsum[sl][1] + s_wsum[sl][2] + s_wsum[sl][3]));
+        }
+        if (k < 0) {
+            cur = nxt;
+            continue;
+        }
+        // emit phase of group g from slot se
+        const int64_t g = k * W + w;
+        const int se = (int)(k & 1);
+        if (k + 1 >= R)
+            __syncthreads();                           // no count phase this round: order the last emit first
+        const int64_t G0 = g * (int64_t)kBfGroup;
+        const int valid = (int)((n - G0) < kBfGroup ? (n - G0) : kBfGroup);
+        const bool last_group = G0 + kBfGroup >= n;
+        const int c_in = s_halo[se];
+        int wsum = 0, wmax = c_in, grp_d = 0, grp_end = c_in;
+#pragma unroll
+        for (int v = 0; v < kWaves; ++v) {
+            const int ws = s_wsum[se][v], wm = s_wmax[se][v];
+            if (v < wave) {
+                wsum += ws;
+                wmax = wm > wmax ? wm : wmax;
+            }
+            grp_d += ws;
+            grp_end = wm > grp_end ? wm : grp_end;
+        }
+        const int le_ex = wave_shr1_i32(cur.le, kNegInf);
+        const int d_loc = wsum + cur.ds - cur.d;
+        const int prev_end = le_ex > wmax ? le_ex : wmax;
+        int c_i = prev_end - b;
+        c_i = c_i < 0 ? 0 : c_i;
+        int c_next = grp_end - kBfGroup;
+        c_next = (c_next < 0 || last_group) ? 0 : c_next;
+        const int64_t out_len = (int64_t)valid + c_next - c_in + grp_d;
+        const int jend = (valid - b) < kBfSeg ? ((valid - b) > 0 ? valid - b : 0) : kBfSeg;
+        int j = c_i < jend ? c_i : jend;
+        int64_t o = (int64_t)(b + j - c_in) + d_loc;   // my first output byte
+        uint64_t M = j >= 64 ? 0ull : (cur.occ & (~0ull << j));
+        const uint8_t *const sp = s_in + se * kIn + kHalo + b;
+        const bool staged = p.write_output && out_len + 32 <= kFusOutCap;
+        // the group's prefix (its output's offset from its input position); with the
+        // image it is waited for after the image is built
+        auto wait_prefix = [&]() -> bool {
+            if (tid == 0) {
+                uint64_t v = pv;
+                int spins = 0;
+                while ((v & kTagMask) != tag) {
+                    if (sweep_should_give_up(a, tag, ++spins)) {
+                        st_agent(a.flag, tag | 1);
+                        s_abort = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                    v = ld_agent(a.pref + g);
+                }
+                s_kexcl = sext48(v & kMask48);
+            }
+            __syncthreads();
+            return !s_abort;
+        };
+        auto put_rep = [&](uint8_t *dst, bool lds, uint64_t en) {
+            const int r = fused_entry_rep_len(en);
+            const int kk = fused_entry_index(en);
+            if (r <= 4) {
+                const uint4 rv = {fused_entry_rep4(en), 0u, 0u, 0u};
+                store_tail(dst, rv, r);
+            } else if (r <= 16) {
+                const uint4 rv = t->rep_head[kk];
+                if (r == 16)
+                    __builtin_memcpy(dst, &rv, 16);
+                else
+                    store_tail(dst, rv, r);
+            } else if (lds) {
+                global_to_lds(dst, t->pool + t->rep_off[kk], r);
+            } else {
+                store_from_global(dst, t->pool + t->rep_off[kk], r);
+            }
+        };
+        if (!p.write_output) {
+            if (!wait_prefix())
+                return;
+        } else if (staged) {
+            int wi = 0;
+            for (;;) {
+                const int q = M ? (int)__builtin_ctzll(M) : jend;
+                bf_copy_run(s_out + o, sp + j, q - j);
+                o += q - j;
+                if (q >= jend)
+                    break;
+                const uint64_t en = wi == 0 ? cur.e0 : (wi == 1 ? cur.e1 : fused_probe(t, sp + q, n - (G0 + b + q)));
+                ++wi;
+                put_rep(s_out + o, true, en);
+                o += fused_entry_rep_len(en);
+                j = q + fused_entry_pat_len(en);
+                M = j >= 64 ? 0ull : (M & (~0ull << j));
+                if (j >= jend)
+                    break;
+            }
+            if (!wait_prefix())
+                return;
+            uint8_t *const out_first = p.dst + (G0 + c_in + s_kexcl);
+            const int len = (int)out_len;
+            for (int c = tid * 16; c < len; c += kThreads * 16) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(s_out + c);
+                if (c + 16 <= len)
+                    __builtin_memcpy(out_first + c, &v, 16);
+                else
+                    store_tail(out_first + c, v, len - c);
+            }
+        } else {
+            // output larger than the image: every thread stores its pieces to HBM
+            if (!wait_prefix())
+                return;
+            uint8_t *out = p.dst + (G0 + c_in + s_kexcl) + o;
+            int wi = 0;
+            for (;;) {
+                const int q = M ? (int)__builtin_ctzll(M) : jend;
+                if (q > j)
+                    store_from_lds(out, sp, j, q - j);
+                out += q - j;
+                if (q >= jend)
+                    break;
+                const uint64_t en = wi == 0 ? cur.e0 : (wi == 1 ? cur.e1 : fused_probe(t, sp + q, n - (G0 + b + q)));
+                ++wi;
+                put_rep(out, false, en);
+                out += fused_entry_rep_len(en);
+                j = q + fused_entry_pat_len(en);
+                M = j >= 64 ? 0ull : (M & (~0ull << j));
+                if (j >= jend)
+                    break;
+            }
+            __syncthreads();   // the pieces were read from the slot: done before it is restaged
+        }
+        cur = nxt;
+    }
+}
+
 __global__ void __launch_bounds__(kThreads)
 fused_count_kernel(ReplaceParams p, const FusedTable *t, int64_t first_group)
 {
@@ -4795,6 +5097,17 @@ hipError_t launch_batch_round(const BatchRound &a, const BatchRoundShape &shape,
         const int nd = shape.class_docs[kBatchEmpty];
         hipLaunchKernelGGL(batch_empty_kernel, dim3((unsigned)(nd < 1024 ? nd : 1024)), dim3(kThreads), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_sweep(const ReplaceParams &p, const FusedTable *t, uint64_t *sweep, int spin_limit,
+                              hipStream_t s)
+{
+    const int64_t G = p.num_groups;
+    SweepArgs a{sweep, sweep + G, sweep + 2 * G, sweep + 2 * G + 1, 1, spin_limit};
+    const int64_t grid = std::min(G + 1, resident_grid(fused_sweep_keep_kernel));
+    hipLaunchKernelGGL(fused_sweep_keep_kernel, dim3((unsigned)std::max<int64_t>(grid, 2)), dim3(kThreads), 0, s, p,
+                       t, a);
     return hipGetLastError();
 }
 

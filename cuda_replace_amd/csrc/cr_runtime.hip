@@ -406,6 +406,17 @@ bool table_single_chosen(int device, int64_t n)
     return n >= smin;
 }
 
+// Fused multi-pattern passes of this many bytes or more run as one persistent launch
+// (CR_FUSED_SWEEP=0 keeps them on three launches; read per pass)
+int64_t fused_sweep_min_bytes()
+{
+    const char *off = getenv("CR_FUSED_SWEEP");
+    if (off && atoi(off) == 0)
+        return INT64_MAX;
+    const char *e = getenv("CR_FUSED_SWEEP_MIN");
+    return e ? (int64_t)atoll(e) : (64ll << 20);
+}
+
 int sweep_ahead()
 {
     const char *e = getenv("CR_SWEEP_AHEAD");
@@ -1222,16 +1233,32 @@ int session_apply_fused(Session *s, bool upload)
         };
         if (bounded)
             CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(worst > 0 ? worst : 1), false, s->stream));
-        CK(launch_fused(p, s->d_fused, 0, 0, num_groups, s->stream));
-        CK(launch_fused(p, s->d_fused, 1, 0, num_groups, s->stream));
-        if (!bounded) {
-            int rc = read_total(&out);
-            if (rc != 0)
-                return rc;
-            CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(out > 0 ? out : 1), false, s->stream));
+        // with the output bounded, large passes run as one persistent launch
+        bool swept = false;
+        if (bounded && n >= fused_sweep_min_bytes() && ensure_sweep(s, num_groups) == hipSuccess) {
+            p.dst = s->d_tmp;
+            CK(launch_fused_sweep(p, s->d_fused, s->d_sweep, sweep_spin_limit(), s->stream));
+            CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * num_groups, 8, hipMemcpyDeviceToHost, s->stream));
+            CK(hipStreamSynchronize(s->stream));
+            swept = *(const uint64_t *)(s->h_misc + 96) != (((uint64_t)s->epoch << kEpochShift) | 1u);
+            if (!swept) {
+                // a wait gave up: the three launches under a new epoch
+                CK(next_epoch(s));
+                p.epoch = s->epoch;
+            }
         }
-        p.dst = s->d_tmp;
-        CK(launch_fused(p, s->d_fused, 2, 0, num_groups, s->stream));
+        if (!swept) {
+            CK(launch_fused(p, s->d_fused, 0, 0, num_groups, s->stream));
+            CK(launch_fused(p, s->d_fused, 1, 0, num_groups, s->stream));
+            if (!bounded) {
+                int rc = read_total(&out);
+                if (rc != 0)
+                    return rc;
+                CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(out > 0 ? out : 1), false, s->stream));
+            }
+            p.dst = s->d_tmp;
+            CK(launch_fused(p, s->d_fused, 2, 0, num_groups, s->stream));
+        }
         int rc = read_total(&out);
         if (rc != 0)
             return rc;

@@ -8,8 +8,12 @@ planted across every such boundary (the emit's coverage carry and halo probe),
 dense inputs made only of occurrences, heavy expansion (the unstaged emit) and
 heavy shrink (negative totals), 64-byte patterns, 256 pairs with colliding hash
 keys, and prefix queries after a fused batch (answered for the last pair by a
-replay of the pairs).
+replay of the pairs).  Every test runs both forms of the fused pass: count, spine
+and emit launches (CR_FUSED_SWEEP=0) and the single persistent launch that holds
+each group in LDS from its count to its emit (CR_FUSED_SWEEP_MIN=0 forces it onto
+every size).
 """
+import os
 import random
 
 import pytest
@@ -20,6 +24,23 @@ from cuda_replace_amd import Session, workloads
 pytestmark = pytest.mark.gpu
 
 SIZES = [0, 1, 5, 100, 8191, 8192, 8193, 16383, 16384, 16385, 40000, 200000, 3 << 20]
+
+
+@pytest.fixture(autouse=True, params=["three_launch", "single_launch"])
+def fused_form(request):
+    keys = ("CR_FUSED_SWEEP", "CR_FUSED_SWEEP_MIN")
+    saved = {k: os.environ.get(k) for k in keys}
+    if request.param == "single_launch":
+        os.environ.pop("CR_FUSED_SWEEP", None)
+        os.environ["CR_FUSED_SWEEP_MIN"] = "0"
+    else:
+        os.environ["CR_FUSED_SWEEP"] = "0"
+    yield request.param
+    for k, v in saved.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
 LETTERS = b"abcdefgh"
 REP_ALPHA = b"XYZ0123456789"
 
