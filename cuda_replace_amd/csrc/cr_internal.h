@@ -129,8 +129,8 @@ struct ReplaceParams {
     int32_t r;
     uint32_t pat4;         // first min(L,4) pattern bytes, little-endian
     uint32_t mask4;        // byte mask of those bytes
-    uint32_t qref[4];      // masked-SAD reference words: pattern bytes 4q..4q+3 xor qx's byte,
-                           // 0 past the pattern (L <= 16; the SAD skips zero reference bytes)
+    uint32_t qref[8];      // masked-SAD reference words: pattern bytes 4q..4q+3 xor qx's byte,
+                           // 0 past the pattern (L <= 32; the SAD skips zero reference bytes)
     uint32_t qx;           // a byte value absent from the pattern, in all four bytes
     uint32_t *gsum;        // per-group kept count, or LT table entries per group
     ChunkState *chunks;    // num_chunks + 1 chunk states (or spine block states)

@@ -4063,15 +4063,35 @@ greedy_bitmap_kernel(ReplaceParams p, int64_t words)
 // candidate of the tiles before it.
 constexpr int kBmFan = 64;
 
-// A tile's candidate words in LDS and, for each word index, the first nonzero word
-// at or after it (kTile / 32 when none).
+// A tile's candidate words in LDS, and per word the first nonzero word and the
+// first word that is not all ones after it (kTile / 32 when none) packed beside it,
+// so a walk step reads one 8-byte entry.
 struct BmTile {
     uint32_t w[kTile / 32];
-    uint16_t nnz[kTile / 32 + 1];
+    uint64_t wn[kTile / 32];   // w[i] | (first nonzero > i) << 32 | (first not all ones > i) << 48
+    uint64_t bnz[kTile / 32 / kWave], bnf[kTile / 32 / kWave];   // per 64 words: nonzero / not all ones
 };
 
-// Loads tile t's candidate words (zero past the buffer) and builds nnz; every thread
-// of the block takes part (blockDim.x divides 256).
+// First index >= i whose bit is set in the per-64-word masks b, or kTile / 32.
+__device__ __forceinline__ int bm_next_set(const uint64_t *b, int i)
+{
+    constexpr int W = kTile / 32;
+    if (i >= W)
+        return W;
+    int c = i >> 6;
+    const uint64_t m = b[c] >> (i & 63);
+    if (m)
+        return i + (int)__builtin_ctzll(m);
+    for (++c; c < W / kWave; ++c)
+        if (b[c])
+            return (c << 6) + (int)__builtin_ctzll(b[c]);
+    return W;
+}
+
+// Loads tile t's candidate words (zero past the buffer) and builds the packed
+// entries from wave ballots of the nonzero and not-all-ones words; every thread of
+// the block takes part (blockDim.x is a multiple of 64 dividing 256, so each wave
+// covers whole 64-word spans).
 __device__ __forceinline__ void bm_load_tile(const ReplaceParams &p, int64_t t, BmTile &s)
 {
     constexpr int W = kTile / 32;
@@ -4081,69 +4101,122 @@ __device__ __forceinline__ void bm_load_tile(const ReplaceParams &p, int64_t t, 
         const int64_t wi = t * W + i;
         const uint32_t v = wi < words ? p.bitmap[wi] : 0u;
         s.w[i] = v;
-        s.nnz[i] = v ? (uint16_t)i : (uint16_t)W;
-    }
-    if (tid == 0)
-        s.nnz[W] = (uint16_t)W;
-    __syncthreads();
-    // suffix minimum over the word indices, doubling the reach each round
-    for (int d = 1; d < W; d <<= 1) {
-        uint16_t v[W / 64];
-        int k = 0;
-        for (int i = tid; i < W; i += bd, ++k) {
-            const uint16_t a = s.nnz[i];
-            const uint16_t b = i + d < W ? s.nnz[i + d] : (uint16_t)W;
-            v[k] = a < b ? a : b;
+        const uint64_t bz = __builtin_amdgcn_ballot_w64(v != 0u);
+        const uint64_t bf = __builtin_amdgcn_ballot_w64(v != 0xffffffffu);
+        if ((i & (kWave - 1)) == 0) {
+            s.bnz[i / kWave] = bz;
+            s.bnf[i / kWave] = bf;
         }
-        __syncthreads();
-        k = 0;
-        for (int i = tid; i < W; i += bd, ++k)
-            s.nnz[i] = v[k];
-        __syncthreads();
     }
+    __syncthreads();
+    for (int i = tid; i < W; i += bd)
+        s.wn[i] = (uint64_t)s.w[i] | (uint64_t)bm_next_set(s.bnz, i + 1) << 32 |
+                  (uint64_t)bm_next_set(s.bnf, i + 1) << 48;
+    __syncthreads();
 }
 
-// First candidate at or after tile offset x, or -1.
-__device__ __forceinline__ int bm_first(const BmTile &s, int x)
+// The greedy over tile offsets [x, end) from entry offset x (end <= kTile): kept
+// count, carry past end, last kept; with MARK each kept start's bit is set in kept[].
+// A step takes the
+// first candidate c at or after x and the run of consecutive candidates from c (to
+// the first clear bit, through any all-ones words): the greedy keeps c, c + L, ...
+// while inside the run, ceil(len / L) starts at once, and goes on from the first
+// position the last of them does not cover.  A word with no candidate left moves x
+// to the next nonempty word.  All-'a' input is then one step per walk; a sparse or
+// periodic input one step per kept start.  inv = floor((2^32 - 1) / L) + 1 makes
+// the high half of (len + L - 1) * inv equal ceil(len / L) for len <= kTile.
+template <bool MARK>
+__device__ __forceinline__ void bm_walk(const BmTile &s, int x, int end, int L, uint32_t inv, int *count, int *carry,
+                                        int *last, uint32_t *kept)
 {
     constexpr int W = kTile / 32;
-    if (x >= kTile)
-        return -1;
-    const int wi = x >> 5;
-    const uint32_t m = s.w[wi] & (0xffffffffu << (x & 31));
-    if (m)
-        return (wi << 5) | __builtin_ctz(m);
-    const int j = s.nnz[wi + 1];
-    return j >= W ? -1 : (j << 5) | __builtin_ctz(s.w[j]);
-}
-
-// The greedy over tile offsets [x, end) from entry offset x: kept count, carry past
-// end, last kept; with MARK each kept start's bit is set in kept[].
-template <bool MARK>
-__device__ __forceinline__ void bm_walk(const BmTile &s, int x, int end, int L, int *count, int *carry, int *last,
-                                        uint32_t *kept)
-{
-    int cnt = 0, lst = -1;
-    for (;;) {
-        const int c = bm_first(s, x);
-        if (c < 0 || c >= end)
-            break;
-        if (MARK)
-            kept[c >> 5] |= 1u << (c & 31);
-        ++cnt;
-        lst = c;
-        x = c + L;
+    int cnt = 0, reach = x;
+    while (x < end) {
+        // a volatile LDS read: the whole entry every step rather than a branch
+        // around the read of the packed indices
+        const uint64_t e = *(const volatile __attribute__((address_space(3))) uint64_t *)(&s.wn[x >> 5]);
+        const uint32_t w = (uint32_t)e;
+        const uint32_t m = w >> (x & 31);
+        const int nz = (int)((e >> 32) & 0xffffu);
+        const int nf = (int)(e >> 48);
+        const int c = x + (int)__builtin_ctz(m | 0x80000000u);
+        const int s5 = c & 31;
+        const int rl = (int)__builtin_ctzll(~(uint64_t)(w >> s5));   // <= 32 - s5
+        int run_end = c + rl;
+        if (s5 + rl == 32) {
+            // the run reaches the word's end: on through the all-ones words
+            const uint32_t wf = s.w[nf < W ? nf : W - 1];
+            run_end = nf < W ? (nf << 5) + (int)__builtin_ctzll(~(uint64_t)wf) : kTile;
+        }
+        const int lim = run_end < end ? run_end : end;
+        const bool has = m != 0u && c < end;
+        const int k = has ? (int)__umulhi((uint32_t)(lim - c + L - 1), inv) : 0;
+        if (MARK) {
+            for (int j = 0; j < k; ++j) {
+                const int q = c + j * L;
+                atomicOr(&kept[q >> 5], 1u << (q & 31));
+            }
+        }
+        cnt += k;
+        const int nxt = c + k * L;
+        reach = has ? nxt : reach;
+        x = m != 0u ? (has ? nxt : end) : (nz << 5);
     }
     *count = cnt;
-    *carry = x > end ? x - end : 0;
-    *last = lst;
+    *carry = reach > end ? reach - end : 0;
+    *last = cnt ? reach - L : -1;
 }
 
-// Candidates of tile t from LDS (L <= kHalo), one 32-position word per thread: the
-// head word compared at every position as the count kernels do, then pattern word k
-// (bytes 4k..4k+3, masked past L) at every position, while any lane of the wave
-// still has a candidate.  Dense inputs (every position a candidate) cost
-// ceil(L / 4) word compares per position instead of L byte compares.
+// Occurrence mask of the 32 positions whose bytes start at s_seg (16-byte aligned
+// LDS) for a pattern of 4(NQ-1) < L <= 4 NQ bytes: the masked quad SADs of
+// bf_mask64_qsad over eight windows of four positions.  Reads s_seg[0, 32 + 4 NQ).
+template <int NQ>
+__device__ __forceinline__ uint32_t bm_mask32_qsad(const ReplaceParams &p, const uint8_t *s_seg)
+{
+    uint32_t d[16];
+    const u32x4 *q4 = reinterpret_cast<const u32x4 *>(s_seg);
+    const uint32_t x = p.qx;
+#pragma unroll
+    for (int k = 0; k < (8 + NQ + 3) / 4; ++k) {
+        const u32x4 v = q4[k];
+        d[4 * k] = v.x ^ x;
+        d[4 * k + 1] = v.y ^ x;
+        d[4 * k + 2] = v.z ^ x;
+        d[4 * k + 3] = v.w ^ x;
+    }
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            acc = __builtin_amdgcn_mqsad_pk_u16_u8(((uint64_t)d[k + q + 1] << 32) | d[k + q], p.qref[q], acc);
+        const uint32_t tl = (0x80008000u - (uint32_t)acc) & 0x80008000u;
+        const uint32_t th = (0x80008000u - (uint32_t)(acc >> 32)) & 0x80008000u;
+        m |= (((tl >> 15) | (tl >> 30) | (th >> 13) | (th >> 28)) & 0xfu) << (4 * k);
+    }
+    return m;
+}
+
+__device__ __forceinline__ uint32_t bm_mask32(const ReplaceParams &p, const uint8_t *s_seg, int nq)
+{
+    switch (nq) {
+    case 1: return bm_mask32_qsad<1>(p, s_seg);
+    case 2: return bm_mask32_qsad<2>(p, s_seg);
+    case 3: return bm_mask32_qsad<3>(p, s_seg);
+    case 4: return bm_mask32_qsad<4>(p, s_seg);
+    case 5: return bm_mask32_qsad<5>(p, s_seg);
+    case 6: return bm_mask32_qsad<6>(p, s_seg);
+    case 7: return bm_mask32_qsad<7>(p, s_seg);
+    default: return bm_mask32_qsad<8>(p, s_seg);
+    }
+}
+
+// Candidates of tile t from LDS (L <= kHalo), one 32-position word per thread.  A
+// pattern of up to 32 bytes takes the masked quad SADs (ceil(L / 4) SAD instructions
+// per four positions).  A longer one compares the head word at every position as the
+// count kernels do, then pattern word k (bytes 4k..4k+3, masked past L) at every
+// position, while any lane of the wave still has a candidate.
 __global__ void __launch_bounds__(kThreads) bm_mark_lds_kernel(ReplaceParams p)
 {
     __shared__ __align__(16) uint8_t s_in[GroupShape<1>::kIn];
@@ -4160,7 +4233,6 @@ __global__ void __launch_bounds__(kThreads) bm_mark_lds_kernel(ReplaceParams p)
     const int64_t gseg = t * kTile + (int64_t)seg;
     uint32_t m = 0;
     if (gseg < p.n) {
-        const uint8_t *s_tile = s_in + kHalo;
         const int64_t lo = p.floor - gseg;
         const int64_t hi = p.n - (int64_t)L - gseg;
         if (hi >= 0 && lo <= 31 && lo <= hi) {
@@ -4170,24 +4242,29 @@ __global__ void __launch_bounds__(kThreads) bm_mark_lds_kernel(ReplaceParams p)
         }
     }
     const int words = (L + 3) / 4;
-    for (int k = 0; k < words; ++k) {
-        if (__builtin_amdgcn_ballot_w64(m != 0u) == 0)
-            break;
-        const uint32_t pw = *reinterpret_cast<const uint32_t *>(s_pat + 4 * k);
-        const int left = L - 4 * k;
-        const uint32_t pm = left >= 4 ? 0xffffffffu : (0xffffffffu >> (8 * (4 - left)));
-        const uint32_t *w32 = reinterpret_cast<const uint32_t *>(s_in + kHalo + seg + 4 * k);
-        uint32_t d[9];
+    if (L <= 32) {
+        if (__builtin_amdgcn_ballot_w64(m != 0u) != 0)
+            m &= bm_mask32(p, s_in + kHalo + seg, words);
+    } else {
+        for (int k = 0; k < words; ++k) {
+            if (__builtin_amdgcn_ballot_w64(m != 0u) == 0)
+                break;
+            const uint32_t pw = *reinterpret_cast<const uint32_t *>(s_pat + 4 * k);
+            const int left = L - 4 * k;
+            const uint32_t pm = left >= 4 ? 0xffffffffu : (0xffffffffu >> (8 * (4 - left)));
+            const uint32_t *w32 = reinterpret_cast<const uint32_t *>(s_in + kHalo + seg + 4 * k);
+            uint32_t d[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q)
-            d[q] = w32[q];
-        uint32_t mk = 0;
+            for (int q = 0; q < 9; ++q)
+                d[q] = w32[q];
+            uint32_t mk = 0;
 #pragma unroll
-        for (int j = 31; j >= 0; --j) {
-            const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
-            mk = mk + mk + (uint32_t)(((w ^ pw) & pm) == 0u);
+            for (int j = 31; j >= 0; --j) {
+                const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
+                mk = mk + mk + (uint32_t)(((w ^ pw) & pm) == 0u);
+            }
+            m &= mk;
         }
-        m &= mk;
     }
     if (gseg < p.n)
         p.bitmap[t * (kTile / 32) + tid] = m;
@@ -4229,10 +4306,11 @@ __global__ void __launch_bounds__(kThreads) bm_tables_kernel(ReplaceParams p, ui
         return;
     }
     const int leaf_bytes = kTile / lpt;
+    const uint32_t inv = 0xffffffffu / (uint32_t)p.L + 1u;
     for (int i = tid; i < lpt * states; i += blockDim.x) {
         const int leaf = i / states, st = i - leaf * states;
         int cnt, carry, lst;
-        bm_walk<false>(s, leaf * leaf_bytes + st, (leaf + 1) * leaf_bytes, p.L, &cnt, &carry, &lst, nullptr);
+        bm_walk<false>(s, leaf * leaf_bytes + st, (leaf + 1) * leaf_bytes, p.L, inv, &cnt, &carry, &lst, nullptr);
         tab[(t * lpt + leaf) * states + st] = ((uint32_t)carry << 16) | (uint32_t)cnt;
     }
 }
@@ -4324,9 +4402,17 @@ __global__ void bm_expand_kernel(const uint32_t *in32, const uint64_t *in64, int
     }
 }
 
+// The kept count of the whole pass, written by bm_top_kernel earlier in the stream.
+__device__ __forceinline__ int64_t bm_total(const ReplaceParams &p)
+{
+    return (int64_t)(p.stats->k_total & ((1ull << kEpochShift) - 1));
+}
+
 // Self-overlapping: each leaf of tile t runs its greedy from its entry state (lane
 // = leaf), the tile's bitmap words are rewritten to the kept starts and its record
-// is written.
+// is written.  Only the last tile holding kept starts (the kept before it plus its
+// own equal the pass's total) reports the last kept start: one atomic per pass
+// instead of one per tile on a single address.
 __global__ void __launch_bounds__(kWave) bm_resolve_kernel(ReplaceParams p, const ChunkState *ent, int lpt)
 {
     __shared__ BmTile s;
@@ -4340,14 +4426,18 @@ __global__ void __launch_bounds__(kWave) bm_resolve_kernel(ReplaceParams p, cons
     if (lane == 0)
         s_last = -1;
     __syncthreads();
+    int cnt = 0;
     if (lane < lpt) {
         const int leaf_bytes = kTile / lpt;
         const ChunkState e = ent[t * lpt + lane];
-        int cnt, carry, lst;
-        bm_walk<true>(s, lane * leaf_bytes + e.carry, (lane + 1) * leaf_bytes, p.L, &cnt, &carry, &lst, s_kept);
+        const uint32_t inv = 0xffffffffu / (uint32_t)p.L + 1u;
+        int carry, lst;
+        bm_walk<true>(s, lane * leaf_bytes + e.carry, (lane + 1) * leaf_bytes, p.L, inv, &cnt, &carry, &lst,
+                      s_kept);
         if (lst >= 0)
             atomicMax(&s_last, lst);
     }
+    const int64_t tile_kept = (int64_t)wave_sum_u32((uint32_t)cnt);
     __syncthreads();
     const int64_t words = p.num_tiles * (kTile / 32);
     for (int i = lane; i < kTile / 32; i += kWave) {
@@ -4364,7 +4454,7 @@ __global__ void __launch_bounds__(kWave) bm_resolve_kernel(ReplaceParams p, cons
         ti.carry_in = e.carry;
         ti.mode = kModeBitmap;
         p.info[t] = ti;
-        if (s_last >= 0)
+        if (s_last >= 0 && e.k_base + tile_kept == bm_total(p))
             atomicMax((unsigned long long *)&p.stats->last_kept,
                       (unsigned long long)(((uint64_t)p.epoch << kEpochShift) | (uint64_t)(tstart + s_last + 1)));
     }
@@ -4394,7 +4484,8 @@ __global__ void __launch_bounds__(kThreads) bm_bf_records_kernel(ReplaceParams p
     ti.carry_in = (int32_t)(carry < INT_MAX ? carry : INT_MAX);
     ti.mode = kModeBitmap;
     p.info[t] = ti;
-    if (tlast[t] >= 0)
+    // only the last tile with candidates reports (every candidate is kept)
+    if (tlast[t] >= 0 && (t + 1 == p.num_tiles || ent[t + 1].k_base == bm_total(p)))
         atomicMax((unsigned long long *)&p.stats->last_kept,
                   (unsigned long long)(((uint64_t)p.epoch << kEpochShift) | (uint64_t)(tstart + tlast[t] + 1)));
 }

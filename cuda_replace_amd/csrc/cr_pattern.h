@@ -59,21 +59,21 @@ inline void pattern_head(const uint8_t *pat, int L, uint32_t *pat4, uint32_t *ma
     *mask4 = m;
 }
 
-// Masked-SAD words of a pattern of up to 16 bytes: c is the smallest byte value that
+// Masked-SAD words of a pattern of up to 32 bytes: c is the smallest byte value that
 // does not occur in the pattern, *qx is c in every byte and qref[q] holds pattern
 // bytes 4q..4q+3 xor c (never zero), with zero bytes past the pattern.  A masked SAD
 // of (data xor *qx) against qref[q] skips exactly the bytes past the pattern.
-inline void pattern_qsad(const uint8_t *pat, int L, uint32_t qref[4], uint32_t *qx)
+inline void pattern_qsad(const uint8_t *pat, int L, uint32_t qref[8], uint32_t *qx)
 {
     bool used[256] = {false};
-    const int m = L < 16 ? L : 16;
+    const int m = L < 32 ? L : 32;
     for (int i = 0; i < m; ++i)
         used[pat[i]] = true;
     int c = 0;
     while (c < 255 && used[c])
         ++c;
     *qx = (uint32_t)c * 0x01010101u;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 8; ++q) {
         uint32_t w = 0;
         for (int i = 0; i < 4; ++i) {
             const int k = 4 * q + i;

@@ -105,3 +105,23 @@ def test_parallel_equals_serial_greedy(lib, serial_env):
         serial_env["CR_BITMAP_SERIAL"] = "1"
         ser = lib.unified(src, pat, rep)
         assert par == ser == src.replace(pat, rep), len(pat)
+
+
+def test_last_kept_from_the_last_tile_with_kept_starts(lib):
+    """The last kept start is reported by the one tile after which nothing is kept:
+    occurrences only in early tiles, only in the final tile, and none at all."""
+    rng = np.random.default_rng(5)
+    base = bytes(rng.integers(98, 123, size=40 * TILE + 9).astype(np.uint8))   # no 'a'
+    bf = bytes(rng.integers(97, 123, size=300).astype(np.uint8)) + b"!"
+    for pat in (b"a" * 17, b"a" * 40, bf):
+        L = len(pat)
+        for where in ([0], [3 * TILE - 5, 7 * TILE + 1], [len(base) - L], [5, len(base) - L - 3], []):
+            b = bytearray(base)
+            for pos in where:
+                b[pos:pos + L] = pat
+            src = bytes(b)
+            for rep in (b"", b"R" * 33):
+                with Session(lib, src) as s:
+                    n2, last = s.apply_seeded(pat, rep, 0)
+                    want, want_last = ro.apply_seeded(src, pat, rep, 0)
+                    assert s.result() == want and n2 == len(want) and last == want_last, (L, where, len(rep))
