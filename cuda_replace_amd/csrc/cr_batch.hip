@@ -263,6 +263,7 @@ int run_other(Engine *E, const DocState &ds, const uint8_t *pat, int L, const ui
     p.L = L;
     p.r = r;
     pattern_head(pat, L, &p.pat4, &p.mask4);
+    pattern_qsad(pat, L, p.qref, &p.qx);
     p.stats = reinterpret_cast<LaunchStats *>(E->d_other.p + pr);
     p.num_tiles = tiles;
     p.num_groups = groups;

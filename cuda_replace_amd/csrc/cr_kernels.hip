@@ -5020,6 +5020,10 @@ hipError_t launch_bitmap_select(const ReplaceParams &p, bool border_free, uint8_
     auto t64 = [&](int l) { return l == 0 ? nullptr : reinterpret_cast<uint64_t *>(scratch + pl.tab_off[l]); };
     auto ent = [&](int l) { return reinterpret_cast<ChunkState *>(scratch + pl.ent_off[l]); };
     int32_t *tlast = border_free ? reinterpret_cast<int32_t *>(scratch + pl.tlast_off) : nullptr;
+    // the mark of a pattern of up to 32 bytes takes the masked-SAD words (pattern_qsad),
+    // whose first word is never zero once set
+    if (p.L <= 32 && p.qref[0] == 0u)
+        return hipErrorInvalidValue;
     if (p.L <= kHalo)
         hipLaunchKernelGGL(bm_mark_lds_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, s, p);
     else {
