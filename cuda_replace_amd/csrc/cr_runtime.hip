@@ -1222,14 +1222,18 @@ int session_apply_fused(Session *s, bool upload)
         const int64_t worst = s->fused_max_grow <= 0 ? n : n + (n / min_len) * s->fused_max_grow;
         const int64_t afford = 4 * n > (256ll << 20) ? 4 * n : (256ll << 20);
         const bool bounded = worst <= afford;
-        auto read_total = [&](int64_t *total) -> int {
-            CK(hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s->stream));
-            CK(hipStreamSynchronize(s->stream));
+        // the signed total length change, from the stats block copied to h_misc + 64
+        auto parse_total = [&](int64_t *total) -> int {
             const uint64_t kt = ((const LaunchStats *)(s->h_misc + 64))->k_total;
             if ((uint32_t)(kt >> kEpochShift) != s->epoch)
                 return (int)hipErrorUnknown;
             *total = n + ((int64_t)(kt << (64 - kEpochShift)) >> (64 - kEpochShift));   // sign-extend
             return 0;
+        };
+        auto read_total = [&](int64_t *total) -> int {
+            CK(hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s->stream));
+            CK(hipStreamSynchronize(s->stream));
+            return parse_total(total);
         };
         if (bounded)
             CK(grow((void **)&s->d_tmp, &s->cap_tmp, (size_t)(worst > 0 ? worst : 1), false, s->stream));
@@ -1238,7 +1242,9 @@ int session_apply_fused(Session *s, bool upload)
         if (bounded && n >= fused_sweep_min_bytes() && ensure_sweep(s, num_groups) == hipSuccess) {
             p.dst = s->d_tmp;
             CK(launch_fused_sweep(p, s->d_fused, s->d_sweep, sweep_spin_limit(), s->stream));
+            // the give-up flag and the stats block under one synchronization
             CK(hipMemcpyAsync(s->h_misc + 96, s->d_sweep + 2 * num_groups, 8, hipMemcpyDeviceToHost, s->stream));
+            CK(hipMemcpyAsync(s->h_misc + 64, s->d_stats, sizeof(LaunchStats), hipMemcpyDeviceToHost, s->stream));
             CK(hipStreamSynchronize(s->stream));
             swept = *(const uint64_t *)(s->h_misc + 96) != (((uint64_t)s->epoch << kEpochShift) | 1u);
             if (!swept) {
@@ -1259,7 +1265,7 @@ int session_apply_fused(Session *s, bool upload)
             p.dst = s->d_tmp;
             CK(launch_fused(p, s->d_fused, 2, 0, num_groups, s->stream));
         }
-        int rc = read_total(&out);
+        int rc = swept ? parse_total(&out) : read_total(&out);
         if (rc != 0)
             return rc;
         if (bounded && out > worst)
