@@ -4088,43 +4088,54 @@ __device__ __forceinline__ int bm_next_set(const uint64_t *b, int i)
     return W;
 }
 
-// Loads tile t's candidate words (zero past the buffer) and builds the packed
-// entries from wave ballots of the nonzero and not-all-ones words; every thread of
-// the block takes part (blockDim.x is a multiple of 64 dividing 256, so each wave
-// covers whole 64-word spans).
-__device__ __forceinline__ void bm_load_tile(const ReplaceParams &p, int64_t t, BmTile &s)
+// Word i of the tile (v) into s, with the wave ballots of the nonzero and the
+// not-all-ones words; every lane of the wave calls it for the same 64-word span.
+__device__ __forceinline__ void bm_put_word(BmTile &s, int i, uint32_t v)
+{
+    s.w[i] = v;
+    const uint64_t bz = __builtin_amdgcn_ballot_w64(v != 0u);
+    const uint64_t bf = __builtin_amdgcn_ballot_w64(v != 0xffffffffu);
+    if ((i & (kWave - 1)) == 0) {
+        s.bnz[i / kWave] = bz;
+        s.bnf[i / kWave] = bf;
+    }
+}
+
+// The packed entries from the words and ballots (after every bm_put_word).
+__device__ __forceinline__ void bm_finish_tile(BmTile &s)
 {
     constexpr int W = kTile / 32;
-    const int tid = threadIdx.x, bd = blockDim.x;
-    const int64_t words = p.num_tiles * W;
-    for (int i = tid; i < W; i += bd) {
-        const int64_t wi = t * W + i;
-        const uint32_t v = wi < words ? p.bitmap[wi] : 0u;
-        s.w[i] = v;
-        const uint64_t bz = __builtin_amdgcn_ballot_w64(v != 0u);
-        const uint64_t bf = __builtin_amdgcn_ballot_w64(v != 0xffffffffu);
-        if ((i & (kWave - 1)) == 0) {
-            s.bnz[i / kWave] = bz;
-            s.bnf[i / kWave] = bf;
-        }
-    }
     __syncthreads();
-    for (int i = tid; i < W; i += bd)
+    for (int i = threadIdx.x; i < W; i += blockDim.x)
         s.wn[i] = (uint64_t)s.w[i] | (uint64_t)bm_next_set(s.bnz, i + 1) << 32 |
                   (uint64_t)bm_next_set(s.bnf, i + 1) << 48;
     __syncthreads();
 }
 
+// Loads tile t's candidate words (zero past the buffer) and builds the packed
+// entries; every thread of the block takes part (blockDim.x is a multiple of 64
+// dividing 256, so each wave covers whole 64-word spans).
+__device__ __forceinline__ void bm_load_tile(const ReplaceParams &p, int64_t t, BmTile &s)
+{
+    constexpr int W = kTile / 32;
+    const int64_t words = p.num_tiles * W;
+    for (int i = threadIdx.x; i < W; i += blockDim.x) {
+        const int64_t wi = t * W + i;
+        bm_put_word(s, i, wi < words ? p.bitmap[wi] : 0u);
+    }
+    bm_finish_tile(s);
+}
+
 // The greedy over tile offsets [x, end) from entry offset x (end <= kTile): kept
 // count, carry past end, last kept; with MARK each kept start's bit is set in kept[].
-// A step takes the
-// first candidate c at or after x and the run of consecutive candidates from c (to
-// the first clear bit, through any all-ones words): the greedy keeps c, c + L, ...
-// while inside the run, ceil(len / L) starts at once, and goes on from the first
-// position the last of them does not cover.  A word with no candidate left moves x
-// to the next nonempty word.  All-'a' input is then one step per walk; a sparse or
-// periodic input one step per kept start.  inv = floor((2^32 - 1) / L) + 1 makes
-// the high half of (len + L - 1) * inv equal ceil(len / L) for len <= kTile.
+// A step takes the first candidate c at or after x and the run of consecutive
+// candidates from c (to the first clear bit, through any all-ones words): the
+// greedy keeps c, c + L, ... while inside the run, ceil(len / L) starts at once,
+// and goes on from the first position the last of them does not cover.  A word
+// with no candidate left moves x to the next nonempty word.  All-'a' input is then
+// one step per walk; a sparse or periodic input one step per kept start.
+// inv = floor((2^32 - 1) / L) + 1 makes the high half of (len + L - 1) * inv equal
+// ceil(len / L) for len <= kTile.
 template <bool MARK>
 __device__ __forceinline__ void bm_walk(const BmTile &s, int x, int end, int L, uint32_t inv, int *count, int *carry,
                                         int *last, uint32_t *kept)
@@ -4212,12 +4223,32 @@ __device__ __forceinline__ uint32_t bm_mask32(const ReplaceParams &p, const uint
     }
 }
 
+// The leaf tables of tile t (self-overlapping pattern; the tile's words in s): entry
+// st of a leaf = carry_out << 16 | kept (kept <= 8192, carry < L <= 256) for the
+// greedy over the leaf whose first st bytes are covered.  Threads take (leaf, entry
+// state) pairs in turn.
+__device__ __forceinline__ void bm_leaf_tables(const ReplaceParams &p, const BmTile &s, int64_t t, uint32_t *tab,
+                                               int states, int lpt)
+{
+    const int leaf_bytes = kTile / lpt;
+    const uint32_t inv = 0xffffffffu / (uint32_t)p.L + 1u;
+    for (int i = threadIdx.x; i < lpt * states; i += blockDim.x) {
+        const int leaf = i / states, st = i - leaf * states;
+        int cnt, carry, lst;
+        bm_walk<false>(s, leaf * leaf_bytes + st, (leaf + 1) * leaf_bytes, p.L, inv, &cnt, &carry, &lst, nullptr);
+        tab[(t * lpt + leaf) * states + st] = ((uint32_t)carry << 16) | (uint32_t)cnt;
+    }
+}
+
 // Candidates of tile t from LDS (L <= kHalo), one 32-position word per thread.  A
 // pattern of up to 32 bytes takes the masked quad SADs (ceil(L / 4) SAD instructions
 // per four positions).  A longer one compares the head word at every position as the
 // count kernels do, then pattern word k (bytes 4k..4k+3, masked past L) at every
-// position, while any lane of the wave still has a candidate.
-__global__ void __launch_bounds__(kThreads) bm_mark_lds_kernel(ReplaceParams p)
+// position, while any lane of the wave still has a candidate.  With TABLES (a
+// self-overlapping pattern) the same workgroup then builds the tile's leaf tables
+// from the words it holds, instead of a second kernel reading them back.
+template <bool TABLES>
+__global__ void __launch_bounds__(kThreads) bm_mark_lds_kernel(ReplaceParams p, uint32_t *tab, int states, int lpt)
 {
     __shared__ __align__(16) uint8_t s_in[GroupShape<1>::kIn];
     __shared__ __align__(16) uint8_t s_pat[kMaxSinglePassPat + 4];
@@ -4266,10 +4297,15 @@ __global__ void __launch_bounds__(kThreads) bm_mark_lds_kernel(ReplaceParams p)
             m &= mk;
         }
     }
-    if (gseg < p.n)
-        p.bitmap[t * (kTile / 32) + tid] = m;
-    else
-        p.bitmap[t * (kTile / 32) + tid] = 0u;
+    if (gseg >= p.n)
+        m = 0u;
+    p.bitmap[t * (kTile / 32) + tid] = m;
+    if constexpr (TABLES) {
+        __shared__ BmTile s;
+        bm_put_word(s, tid, m);
+        bm_finish_tile(s);
+        bm_leaf_tables(p, s, t, tab, states, lpt);
+    }
 }
 
 // Level-0 tables, one per leaf (a tile holds lpt leaves of kTile / lpt bytes): entry s
@@ -4305,14 +4341,7 @@ __global__ void __launch_bounds__(kThreads) bm_tables_kernel(ReplaceParams p, ui
         }
         return;
     }
-    const int leaf_bytes = kTile / lpt;
-    const uint32_t inv = 0xffffffffu / (uint32_t)p.L + 1u;
-    for (int i = tid; i < lpt * states; i += blockDim.x) {
-        const int leaf = i / states, st = i - leaf * states;
-        int cnt, carry, lst;
-        bm_walk<false>(s, leaf * leaf_bytes + st, (leaf + 1) * leaf_bytes, p.L, inv, &cnt, &carry, &lst, nullptr);
-        tab[(t * lpt + leaf) * states + st] = ((uint32_t)carry << 16) | (uint32_t)cnt;
-    }
+    bm_leaf_tables(p, s, t, tab, states, lpt);
 }
 
 // Entry e of a table at a level: level 0 packs carry << 16 | count in 32 bits,
@@ -5024,17 +5053,24 @@ hipError_t launch_bitmap_select(const ReplaceParams &p, bool border_free, uint8_
     // whose first word is never zero once set
     if (p.L <= 32 && p.qref[0] == 0u)
         return hipErrorInvalidValue;
-    if (p.L <= kHalo)
-        hipLaunchKernelGGL(bm_mark_lds_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, s, p);
-    else {
-        const hipError_t e = launch_bitmap_mark(p, s);
-        if (e != hipSuccess)
-            return e;
+    if (p.L <= kHalo && !border_free) {
+        // candidates and leaf tables in one pass over each tile
+        hipLaunchKernelGGL(bm_mark_lds_kernel<true>, dim3((unsigned)tiles), dim3(kThreads), 0, s, p, t32(0), st, lpt);
+    } else {
+        if (p.L <= kHalo) {
+            hipLaunchKernelGGL(bm_mark_lds_kernel<false>, dim3((unsigned)tiles), dim3(kThreads), 0, s, p, nullptr, 0,
+                               0);
+        } else {
+            const hipError_t e = launch_bitmap_mark(p, s);
+            if (e != hipSuccess)
+                return e;
+        }
+        // one wave per tile when the states fit one (more tiles in flight per unit:
+        // the walks are latency-bound); border-free tiles reduce with the full block
+        const unsigned tab_threads = (!border_free && st * lpt <= kWave) ? (unsigned)kWave : (unsigned)kThreads;
+        hipLaunchKernelGGL(bm_tables_kernel, dim3((unsigned)tiles), dim3(tab_threads), 0, s, p, t32(0), st, lpt,
+                           tlast);
     }
-    // one wave per tile when the states fit one (more tiles in flight per unit: the
-    // walks are latency-bound); border-free tiles reduce with the full block
-    const unsigned tab_threads = (!border_free && st * lpt <= kWave) ? (unsigned)kWave : (unsigned)kThreads;
-    hipLaunchKernelGGL(bm_tables_kernel, dim3((unsigned)tiles), dim3(tab_threads), 0, s, p, t32(0), st, lpt, tlast);
     const unsigned comp_threads = (unsigned)(st >= kThreads ? kThreads : (st + kWave - 1) / kWave * kWave);
     for (int l = 1; l < pl.levels; ++l)
         hipLaunchKernelGGL(bm_compose_kernel, dim3((unsigned)pl.n[l]), dim3(comp_threads), 0, s, t32(l - 1),
