@@ -4360,21 +4360,55 @@ __device__ __forceinline__ void bm_entry(const uint32_t *t32, const uint64_t *t6
     }
 }
 
+// Tables of at most this many states are staged in LDS by the compose and expand
+// kernels (64 children x states x 8 bytes <= 32 KiB of dynamic LDS): all of a
+// block's child entries are loaded at once, then the walks run over LDS instead of a
+// chain of 64 dependent global loads.
+constexpr int kBmStageStates = 64;
+
+size_t bm_stage_bytes(int states)
+{
+    return states <= kBmStageStates ? (size_t)kBmFan * (size_t)states * sizeof(uint64_t) : 0;
+}
+
+// Block b's children's entries (first .. first + nch - 1, every state) into s_t as
+// carry << 32 | count.
+__device__ __forceinline__ void bm_stage_children(const uint32_t *in32, const uint64_t *in64, int64_t first,
+                                                  int nch, int states, uint64_t *s_t)
+{
+    for (int i = threadIdx.x; i < nch * states; i += blockDim.x) {
+        uint32_t c;
+        uint64_t k;
+        bm_entry(in32, in64, first * states + i, &c, &k);
+        s_t[i] = (uint64_t)c << 32 | k;
+    }
+    __syncthreads();
+}
+
 // out[b] = in[64b] then in[64b+1] ... then in[64b+63] (composition), per state.
 __global__ void __launch_bounds__(kThreads) bm_compose_kernel(const uint32_t *in32, const uint64_t *in64,
                                                                int64_t n_in, int states, uint64_t *out)
 {
+    extern __shared__ uint64_t s_t[];
     const int64_t b = blockIdx.x;
+    const int64_t first = b * kBmFan;
+    const int nch = (int)(n_in - first < kBmFan ? n_in - first : kBmFan);
+    const bool staged = states <= kBmStageStates;
+    if (staged)
+        bm_stage_children(in32, in64, first, nch, states, s_t);
     for (int s0 = threadIdx.x; s0 < states; s0 += blockDim.x) {
         uint32_t st = (uint32_t)s0;
         uint64_t cnt = 0;
-        for (int j = 0; j < kBmFan; ++j) {
-            const int64_t i = b * kBmFan + j;
-            if (i >= n_in)
-                break;
+        for (int j = 0; j < nch; ++j) {
             uint32_t c;
             uint64_t k;
-            bm_entry(in32, in64, i * states + st, &c, &k);
+            if (staged) {
+                const uint64_t e = s_t[j * states + (int)st];
+                c = (uint32_t)(e >> 32);
+                k = e & 0xffffffffull;
+            } else {
+                bm_entry(in32, in64, (first + j) * states + st, &c, &k);
+            }
             cnt += k;
             st = c;
         }
@@ -4405,27 +4439,37 @@ __global__ void bm_top_kernel(const uint32_t *in32, const uint64_t *in64, int64_
     stats->k_total = ((uint64_t)epoch << kEpochShift) | ((uint64_t)k & ((1ull << kEpochShift) - 1));
 }
 
-// Children of parent b get their entries by walking its 64 tables from b's entry.
+// Children of parent b get their entries by walking its 64 tables from b's entry
+// (thread 0, over the staged entries when they fit LDS).
 __global__ void bm_expand_kernel(const uint32_t *in32, const uint64_t *in64, int64_t n_in, int states,
                                  const ChunkState *parent, ChunkState *child)
 {
+    extern __shared__ uint64_t s_t[];
+    const int64_t b = blockIdx.x;
+    const int64_t first = b * kBmFan;
+    const int nch = (int)(n_in - first < kBmFan ? n_in - first : kBmFan);
+    const bool staged = states <= kBmStageStates;
+    if (staged)
+        bm_stage_children(in32, in64, first, nch, states, s_t);
     if (threadIdx.x != 0)
         return;
-    const int64_t b = blockIdx.x;
     int64_t k = parent[b].k_base;
     uint32_t st = (uint32_t)parent[b].carry;
-    for (int j = 0; j < kBmFan; ++j) {
-        const int64_t i = b * kBmFan + j;
-        if (i >= n_in)
-            break;
+    for (int j = 0; j < nch; ++j) {
         ChunkState cs;
         cs.k_base = k;
         cs.carry = (int)st;
         cs.pad = 0;
-        child[i] = cs;
+        child[first + j] = cs;
         uint32_t c;
         uint64_t kk;
-        bm_entry(in32, in64, i * states + st, &c, &kk);
+        if (staged) {
+            const uint64_t e = s_t[j * states + (int)st];
+            c = (uint32_t)(e >> 32);
+            kk = e & 0xffffffffull;
+        } else {
+            bm_entry(in32, in64, (first + j) * states + st, &c, &kk);
+        }
         k += (int64_t)kk;
         st = c;
     }
@@ -5072,15 +5116,16 @@ hipError_t launch_bitmap_select(const ReplaceParams &p, bool border_free, uint8_
                            tlast);
     }
     const unsigned comp_threads = (unsigned)(st >= kThreads ? kThreads : (st + kWave - 1) / kWave * kWave);
+    const size_t stage = bm_stage_bytes(st);
     for (int l = 1; l < pl.levels; ++l)
-        hipLaunchKernelGGL(bm_compose_kernel, dim3((unsigned)pl.n[l]), dim3(comp_threads), 0, s, t32(l - 1),
+        hipLaunchKernelGGL(bm_compose_kernel, dim3((unsigned)pl.n[l]), dim3(comp_threads), stage, s, t32(l - 1),
                            t64(l - 1), pl.n[l - 1], st, t64(l));
     const int top = pl.levels - 1;
     hipLaunchKernelGGL(bm_top_kernel, dim3(1), dim3(1), 0, s, t32(top), t64(top), pl.n[top], st, ent(top), p.stats,
                        p.epoch);
     for (int l = top; l >= 1; --l)
-        hipLaunchKernelGGL(bm_expand_kernel, dim3((unsigned)pl.n[l]), dim3(1), 0, s, t32(l - 1), t64(l - 1),
-                           pl.n[l - 1], st, ent(l), ent(l - 1));
+        hipLaunchKernelGGL(bm_expand_kernel, dim3((unsigned)pl.n[l]), dim3(stage ? kWave : 1), stage, s, t32(l - 1),
+                           t64(l - 1), pl.n[l - 1], st, ent(l), ent(l - 1));
     if (border_free)
         hipLaunchKernelGGL(bm_bf_records_kernel, dim3((unsigned)((tiles + kThreads - 1) / kThreads)), dim3(kThreads),
                            0, s, p, ent(0), tlast);
