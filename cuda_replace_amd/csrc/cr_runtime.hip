@@ -431,15 +431,12 @@ int sweep_spin_limit()
     return v < 1 ? 1 : v;
 }
 
-// The untuned form of a border-free pass of n bytes: from 256 MiB the single launch
-// holding groups in LDS, from CR_SWEEP_MIN (64 MiB) the single launch re-staging
-// them, below it the three launches of the same kernels.
+// The untuned form of a border-free pass of n bytes: from CR_SWEEP_MIN (64 MiB) the
+// single launch holding groups in LDS, below it the three launches of the
+// re-staging launch's kernels.
 int default_bf_variant(int64_t n)
 {
-    const int64_t smin = sweep_min_bytes();
-    if (n < smin)
-        return kBfLean;
-    return n >= (256ll << 20) ? kBfKeep : kBfSweep;
+    return n < sweep_min_bytes() ? kBfLean : kBfKeep;
 }
 
 // The form of a border-free pass (BfVariant); *tune is set when this pass should
