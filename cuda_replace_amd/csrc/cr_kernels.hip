@@ -433,6 +433,101 @@ __device__ __forceinline__ void load_group(const ReplaceParams &p, int64_t t0, i
                                          (lds_void *)(s_in + kHalo + kGroupBytes), 4, 0, 0);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Occurrence mask of the 32 positions whose bytes start at s_seg (16-byte aligned
+// LDS) for a pattern of 4(NQ-1) < L <= 4 NQ bytes: the masked quad SADs of
+// bf_mask64_qsad (below) over eight windows of four positions, against the reference words
+// qref and the absent byte qx (pattern_qsad).  Reads s_seg[0, 32 + 4 NQ).
+template <int NQ>
+__device__ __forceinline__ uint32_t mask32_qsad(const uint32_t *qref, uint32_t qx, const uint8_t *s_seg)
+{
+    uint32_t d[16];
+    const u32x4 *q4 = reinterpret_cast<const u32x4 *>(s_seg);
+#pragma unroll
+    for (int k = 0; k < (8 + NQ + 3) / 4; ++k) {
+        const u32x4 v = q4[k];
+        d[4 * k] = v.x ^ qx;
+        d[4 * k + 1] = v.y ^ qx;
+        d[4 * k + 2] = v.z ^ qx;
+        d[4 * k + 3] = v.w ^ qx;
+    }
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            acc = __builtin_amdgcn_mqsad_pk_u16_u8(((uint64_t)d[k + q + 1] << 32) | d[k + q], qref[q], acc);
+        const uint32_t tl = (0x80008000u - (uint32_t)acc) & 0x80008000u;
+        const uint32_t th = (0x80008000u - (uint32_t)(acc >> 32)) & 0x80008000u;
+        m |= (((tl >> 15) | (tl >> 30) | (th >> 13) | (th >> 28)) & 0xfu) << (4 * k);
+    }
+    return m;
+}
+
+// Positions of [gseg, gseg+32) where an occurrence may start: floor <= pos <= n - L.
+__device__ __forceinline__ uint32_t range_mask32(const ReplaceParams &p, int64_t gseg)
+{
+    const int64_t lo = p.floor - gseg;
+    const int64_t hi = p.n - (int64_t)p.L - gseg;
+    if (hi < 0 || lo > 31 || lo > hi)
+        return 0u;
+    const int a = lo < 0 ? 0 : (int)lo;
+    const int b = hi > 31 ? 31 : (int)hi;
+    return (b == 31 ? 0xffffffffu : ((1u << (b + 1)) - 1u)) & (0xffffffffu << a);
+}
+
+// The masked-SAD words of the pattern in s_pat (L <= 16; as pattern_qsad on the
+// host), computed by every wave for itself: lane v tests byte value v, and the
+// smallest value absent from the pattern (below 17) is the first clear bit of the
+// ballot.
+struct QsadWords {
+    uint32_t qref[4];
+    uint32_t qx;
+};
+
+__device__ __forceinline__ QsadWords qsad_words_lds(const uint8_t *s_pat, int L)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    bool present = false;
+    for (int i = 0; i < L; ++i)
+        present |= s_pat[i] == lane;
+    const uint32_t c = (uint32_t)__builtin_ctzll(~__builtin_amdgcn_ballot_w64(present));
+    QsadWords q;
+    q.qx = c * 0x01010101u;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int k = 4 * w + b;
+            if (k < L)
+                v |= (uint32_t)(s_pat[k] ^ (uint8_t)c) << (8 * b);
+        }
+        q.qref[w] = v;
+    }
+    return q;
+}
+
+// Candidate mask of the 32 positions [gseg, gseg+32) whose bytes start at s_seg
+// (16-byte aligned LDS) for a pattern of up to 16 bytes, from masked quad SADs:
+// every position costs the same whatever the data (the byte loop of cand_mask runs
+// once per head match, all of them on dense input).
+__device__ __forceinline__ uint32_t cand_mask_qsad(const ReplaceParams &p, const QsadWords &qw, const uint8_t *s_seg,
+                                                   int64_t gseg)
+{
+    const uint32_t rm = range_mask32(p, gseg);
+    if (!rm)
+        return 0u;
+    switch ((p.L + 3) >> 2) {
+    case 1: return rm & mask32_qsad<1>(qw.qref, qw.qx, s_seg);
+    case 2: return rm & mask32_qsad<2>(qw.qref, qw.qx, s_seg);
+    case 3: return rm & mask32_qsad<3>(qw.qref, qw.qx, s_seg);
+    default: return rm & mask32_qsad<4>(qw.qref, qw.qx, s_seg);
+    }
+}
+
 // Candidate mask of the 32 positions [gseg, gseg+32) whose bytes start at
 // s_tile + seg in LDS: bit j set iff the pattern occurs at gseg + j and
 // floor <= gseg + j <= n - L.  Reads up to L - 1 bytes past the segment.
@@ -454,15 +549,7 @@ __device__ __forceinline__ uint32_t cand_mask(const ReplaceParams &p, const uint
         const uint32_t w = (j % 4 == 0) ? d[j / 4] : __builtin_amdgcn_alignbyte(d[j / 4 + 1], d[j / 4], j % 4);
         m = m + m + (uint32_t)(((w ^ pat4) & mask4) == 0u);
     }
-    const int64_t lo = p.floor - gseg;
-    const int64_t hi = p.n - (int64_t)L - gseg;
-    uint32_t rm = 0;
-    if (hi >= 0 && lo <= 31 && lo <= hi) {
-        const int a = lo < 0 ? 0 : (int)lo;
-        const int b = hi > 31 ? 31 : (int)hi;
-        rm = (b == 31 ? 0xffffffffu : ((1u << (b + 1)) - 1u)) & (0xffffffffu << a);
-    }
-    m &= rm;
+    m &= range_mask32(p, gseg);
     if (L > 4 && m) {
         uint32_t mm = m;
         m = 0;
@@ -554,9 +641,10 @@ __device__ __forceinline__ void count_group(const ReplaceParams &p, int64_t g, u
         // carry out of the input's last group is never used.
         const int b0 = tid * kBytesPerThread * GT;
         const int64_t g0 = t0 * kTile;
+        const QsadWords qw = qsad_words_lds(s_pat, p.L);
         // segment masks of tiles that exist are kept for the emit
         auto seg_mask = [&](int b) -> uint32_t {
-            const uint32_t m = cand_mask(p, s_in + kHalo, s_pat, b, g0 + b);
+            const uint32_t m = cand_mask_qsad(p, qw, s_in + kHalo + b, g0 + b);
             if (p.masks && t0 + b / kTile < p.num_tiles)
                 p.masks[(g0 + b) / 32] = m;
             return m;
@@ -1395,7 +1483,6 @@ constexpr int kBfSeg = 64;                            // source bytes per thread
 constexpr int kBfGroup = kThreads * kBfSeg;           // 16384 source bytes per workgroup
 static_assert(kBfGroup == kGroupBytes, "the spine's group size");
 constexpr int kBfOutCap = kBfGroup + 4096;            // LDS image of a group's output
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // wave64 inclusive scans with DPP (row_shr 1/2/4/8 inside rows of 16 lanes, then
 // row_bcast 15 and 31 across rows).  Lanes without a source read the identity.
@@ -4178,35 +4265,10 @@ __device__ __forceinline__ void bm_walk(const BmTile &s, int x, int end, int L, 
     *last = cnt ? reach - L : -1;
 }
 
-// Occurrence mask of the 32 positions whose bytes start at s_seg (16-byte aligned
-// LDS) for a pattern of 4(NQ-1) < L <= 4 NQ bytes: the masked quad SADs of
-// bf_mask64_qsad over eight windows of four positions.  Reads s_seg[0, 32 + 4 NQ).
 template <int NQ>
 __device__ __forceinline__ uint32_t bm_mask32_qsad(const ReplaceParams &p, const uint8_t *s_seg)
 {
-    uint32_t d[16];
-    const u32x4 *q4 = reinterpret_cast<const u32x4 *>(s_seg);
-    const uint32_t x = p.qx;
-#pragma unroll
-    for (int k = 0; k < (8 + NQ + 3) / 4; ++k) {
-        const u32x4 v = q4[k];
-        d[4 * k] = v.x ^ x;
-        d[4 * k + 1] = v.y ^ x;
-        d[4 * k + 2] = v.z ^ x;
-        d[4 * k + 3] = v.w ^ x;
-    }
-    uint32_t m = 0u;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        uint64_t acc = 0;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            acc = __builtin_amdgcn_mqsad_pk_u16_u8(((uint64_t)d[k + q + 1] << 32) | d[k + q], p.qref[q], acc);
-        const uint32_t tl = (0x80008000u - (uint32_t)acc) & 0x80008000u;
-        const uint32_t th = (0x80008000u - (uint32_t)(acc >> 32)) & 0x80008000u;
-        m |= (((tl >> 15) | (tl >> 30) | (th >> 13) | (th >> 28)) & 0xfu) << (4 * k);
-    }
-    return m;
+    return mask32_qsad<NQ>(p.qref, p.qx, s_seg);
 }
 
 __device__ __forceinline__ uint32_t bm_mask32(const ReplaceParams &p, const uint8_t *s_seg, int nq)
