@@ -1317,27 +1317,30 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
                     s_wtab[wave][c] = inc.e[c];
             }
             __syncthreads();
-            Table<LT> wpre = tidentity<LT>();
-            Table<LT> agg = tidentity<LT>();
-            for (int w = 0; w < kWaves; ++w) {
-                Table<LT> wt;
+            // the tile's entry state c_t through the wave totals: the state and kept
+            // count entering my wave, and leaving the tile (each step one LDS read of
+            // the entry at the current state, not a composition of whole tables)
+            uint32_t st = (uint32_t)c_t, kw = 0, s_mine = 0, k_mine = 0;
 #pragma unroll
-                for (int c = 0; c < LT; ++c)
-                    wt.e[c] = s_wtab[w][c];
-                if (w < wave)
-                    wpre = tcompose<LT>(wpre, wt);
-                agg = tcompose<LT>(agg, wt);
+            for (int w = 0; w < kWaves; ++w) {
+                if (w == wave) {
+                    s_mine = st;
+                    k_mine = kw;
+                }
+                const uint32_t e = s_wtab[w][st];
+                kw += e & 0xffffffu;
+                st = e >> 24;
             }
-            const uint32_t me = tsel<LT>(tcompose<LT>(wpre, ex), (uint32_t)c_t);
+            const uint32_t me = tsel<LT>(ex, s_mine);
             c_i = (int)(me >> 24);
-            k_loc = (int)(me & 0xffffffu);
-            c_next = last_tile ? 0 : (int)(tsel<LT>(agg, (uint32_t)c_t) >> 24);
+            k_loc = (int)(k_mine + (me & 0xffffffu));
+            c_next = last_tile ? 0 : (int)st;
             int lst;
             kept = LT == 2 ? greedy32_pair(m, c_i, &lst) : greedy32(m, c_i, L, &lst);
             const int lastq = kept ? seg + lst : kNegInf;
-            // kept starts in the tile from its composed table; the last one only for
-            // the tile's record
-            tile_k = (int)(tsel<LT>(agg, (uint32_t)c_t) & 0xffffffu);
+            // kept starts in the tile from the wave totals; the last one only for the
+            // tile's record
+            tile_k = (int)kw;
             if (p.info) {
                 const int lq = __builtin_amdgcn_readlane(wave_incl_max_i32(lastq), kWave - 1);
                 if (lane == 0)
