@@ -1216,6 +1216,10 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const uint4 rep_v = lds_load16(s_rep);   // the replacement's first 16 bytes
+    // carry tables without the count's masks (batch rounds) rebuild them from SADs
+    QsadWords qw{};
+    if (MODE == kModeTable && !have_masks)
+        qw = qsad_words_lds(s_pat, L);
 
     const int seg = tid * kBytesPerThread;
     for (int i = 0; i < kGroupTiles; ++i) {
@@ -1238,6 +1242,8 @@ __device__ __forceinline__ void emit_group(const ReplaceParams &p, int64_t g, co
 #pragma unroll
             for (int k = 0; k < GT; ++k)
                 m = k == i ? cm[k] : m;
+        } else if constexpr (MODE == kModeTable) {
+            m = cand_mask_qsad(p, qw, s_in + in_off + seg, start + seg);
         } else {
             m = cand_mask(p, s_in + in_off, s_pat, seg, start + seg);
         }
