@@ -4437,7 +4437,7 @@ __device__ __forceinline__ void bm_entry(const uint32_t *t32, const uint64_t *t6
 // chain of 64 dependent global loads.
 constexpr int kBmStageStates = 64;
 
-size_t bm_stage_bytes(int states)
+static size_t bm_stage_bytes(int states)
 {
     return states <= kBmStageStates ? (size_t)kBmFan * (size_t)states * sizeof(uint64_t) : 0;
 }
