@@ -250,10 +250,22 @@ struct DeviceGuard {
 size_t round_up(size_t v, size_t g) { return (v + g - 1) / g * g; }
 
 // grow a device allocation without preserving contents
+// Test hook: CR_FAIL_GROW_OVER=bytes makes every device buffer growth to more than
+// that many bytes fail as an out-of-memory hipMalloc would (read at each growth), so
+// tests can check that a failed growth leaves the session's data and buffers as
+// they were.
+static size_t fail_grow_over()
+{
+    const char *e = getenv("CR_FAIL_GROW_OVER");
+    return e ? (size_t)strtoull(e, nullptr, 10) : SIZE_MAX;
+}
+
 hipError_t grow(void **ptr, size_t *cap, size_t need_bytes, bool zero, hipStream_t s)
 {
     if (*cap >= need_bytes && *ptr)
         return hipSuccess;
+    if (need_bytes > fail_grow_over())
+        return hipErrorOutOfMemory;
     size_t nc = round_up(need_bytes > 0 ? need_bytes : 1, kAllocGranule);
     if (*cap > 0 && nc < *cap + *cap / 2)
         nc = round_up(*cap + *cap / 2, kAllocGranule);
