@@ -3755,6 +3755,43 @@ __device__ __forceinline__ void fused_scanner(const ReplaceParams &p, const Swee
         st_agent(&p.stats->k_total, tag | ((uint64_t)running & kMask48));
 }
 
+// Positions among the 64 whose bytes start at s_seg (16-byte aligned LDS) that hold
+// the byte p.pat4 & 0xff, within [floor, n - L]: per dword, the exact zero-byte test
+// of (d xor f * 0x01010101) leaves bit 7 of each matching byte, and a multiply by
+// 0x00204081 gathers those four bits into the top nibble (each lands on its own bit
+// 28 + i, and no other product term reaches bits 28..31).
+__device__ __forceinline__ uint64_t first_byte64(const ReplaceParams &p, const uint8_t *s_seg, int64_t gpos)
+{
+    const uint32_t f4 = (p.pat4 & 0xffu) * 0x01010101u;
+    const uint4 *q = reinterpret_cast<const uint4 *>(s_seg);
+    uint32_t half[2] = {0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 v = q[k];
+        const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t t = d[i] ^ f4;
+            const uint32_t z = ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t) & 0x80808080u;
+            const int idx = 4 * k + i;
+            half[idx / 8] |= ((z * 0x00204081u) >> 28) << (4 * (idx % 8));
+        }
+    }
+    uint64_t m = ((uint64_t)half[1] << 32) | half[0];
+    const int64_t lo_lim = p.floor - gpos;
+    const int64_t hi_lim = p.n - (int64_t)p.L - gpos;
+    if (lo_lim > 0 || hi_lim < 63) {
+        if (hi_lim < 0 || lo_lim > 63 || lo_lim > hi_lim) {
+            m = 0;
+        } else {
+            const int a = lo_lim < 0 ? 0 : (int)lo_lim;
+            const int b = hi_lim > 63 ? 63 : (int)hi_lim;
+            m &= (b == 63 ? ~0ull : ((2ull << b) - 1ull)) & (~0ull << a);
+        }
+    }
+    return m;
+}
+
 // What a group's count phase hands its emit (per thread).
 struct FusPre {
     uint64_t occ;        // occurrences starting in my 64 bytes
@@ -3820,7 +3857,7 @@ fused_sweep_keep_kernel(ReplaceParams p, const FusedTable *t, SweepArgs a)
             const int64_t G0c = gc * (int64_t)kBfGroup;
             const int64_t gb = G0c + b;
             const uint8_t *sp = slot + kHalo + b;
-            uint64_t m = gb < n ? bf_cand64<false>(p, sp, slot, gb) : 0ull;   // the shared first byte
+            uint64_t m = gb < n ? first_byte64(p, sp, gb) : 0ull;   // the shared first byte
             nxt.occ = 0;
             nxt.e0 = nxt.e1 = 0;
             int d = 0, e = kNegInf, nk = 0;
