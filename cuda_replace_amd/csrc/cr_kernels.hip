@@ -510,6 +510,28 @@ __device__ __forceinline__ QsadWords qsad_words_lds(const uint8_t *s_pat, int L)
     return q;
 }
 
+// Positions of [gseg, gseg+32) (bytes from s_seg, 16-byte aligned LDS) holding the
+// byte p.pat4 & 0xff, within [floor, n - L]: the zero-byte test and flag gathering
+// of first_byte64 on eight dwords.
+__device__ __forceinline__ uint32_t first_byte32(const ReplaceParams &p, const uint8_t *s_seg, int64_t gseg)
+{
+    const uint32_t f4 = (p.pat4 & 0xffu) * 0x01010101u;
+    const uint4 *q = reinterpret_cast<const uint4 *>(s_seg);
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint4 v = q[k];
+        const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t t = d[i] ^ f4;
+            const uint32_t z = ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t) & 0x80808080u;
+            m |= ((z * 0x00204081u) >> 28) << (4 * (4 * k + i));
+        }
+    }
+    return m & range_mask32(p, gseg);
+}
+
 // Candidate mask of the 32 positions [gseg, gseg+32) whose bytes start at s_seg
 // (16-byte aligned LDS) for a pattern of up to 16 bytes, from masked quad SADs:
 // every position costs the same whatever the data (the byte loop of cand_mask runs
@@ -3473,7 +3495,7 @@ __device__ __forceinline__ void fused_count_group(const ReplaceParams &p, const 
             break;
         const int64_t gseg = ti * kTile + seg;
         const uint8_t *sp = s_in + kHalo + i * kTile + seg;
-        uint32_t m = cand_mask(p, s_in + kHalo + i * kTile, s_in, seg, gseg);
+        uint32_t m = first_byte32(p, s_in + kHalo + i * kTile + seg, gseg);   // the shared first byte
         uint32_t occ = 0;   // occurrences (all kept: they never overlap)
         while (m) {
             const int q = __builtin_ctz(m);
@@ -3559,7 +3581,7 @@ __device__ __forceinline__ void fused_emit_group(const ReplaceParams &p, const F
             for (int k = 0; k < kGroupTiles; ++k)
                 m = k == i ? cm[k] : m;
         } else {
-            m = cand_mask(p, s_in + in_off, s_in, seg, start + seg);
+            m = first_byte32(p, s_in + in_off + seg, start + seg);   // the shared first byte
         }
         uint32_t kept = 0;
         int d = 0;
