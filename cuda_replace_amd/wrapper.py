@@ -322,7 +322,18 @@ class Session:
         self.last_batch_fused = False
         if count == 0:
             return self.length()
-        arrs = _pair_arrays(pairs)
+        # the pointer arrays of the last pair list are reused when equal pairs come
+        # again (the library keeps that list's table too): the arrays hold copies,
+        # and the kept key is a snapshot of pairs of bytes, which cannot change
+        key = tuple(pairs)
+        cached = getattr(self, "_batch_arrays", None)
+        if cached is not None and cached[0] == key:
+            arrs = cached[1]
+        else:
+            arrs = _pair_arrays(pairs)
+            immutable = all(type(pr) is tuple and len(pr) == 2 and type(pr[0]) is bytes and type(pr[1]) is bytes
+                            for pr in key)
+            self._batch_arrays = (key, arrs) if immutable else None
         new_len = c_size_t()
         fused = C.c_int(0)
         rc = self.lib.lib.cr_apply_batch_ex(self.h, *arrs[:4], count, C.byref(new_len), 1 if fuse else 0,

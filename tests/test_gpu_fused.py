@@ -218,6 +218,34 @@ def test_repeated_and_changed_pair_lists(lib):
             assert s.result() == ro.apply_batch(src, prs), k
 
 
+def test_pair_list_mutated_between_calls(lib):
+    # the wrapper reuses the pointer arrays of an equal pair list: a list changed in
+    # place, pairs given as lists or bytearrays, and the original list again must
+    # each produce their own result
+    rng = random.Random(49)
+    pairs = entity_list(rng, 5)
+    src = planted(rng, 40000, pairs, density=0.05)
+    with Session(lib, b"") as s:
+        mutated = list(pairs)
+        for want_pairs in (pairs, pairs):
+            s.reset(src)
+            s.apply_batch(want_pairs)
+            assert s.result() == ro.apply_batch(src, want_pairs)
+        mutated[0] = (mutated[0][0], mutated[0][1] + b"Q")
+        for prs in (mutated, [list(p) for p in pairs], [(bytearray(p), bytearray(r)) for p, r in pairs], pairs):
+            want = ro.apply_batch(src, [(bytes(p), bytes(r)) for p, r in prs])
+            s.reset(src)
+            s.apply_batch(prs)
+            assert s.result() == want
+        edit = [(bytearray(p), bytearray(r)) for p, r in pairs]
+        s.reset(src)
+        s.apply_batch(edit)
+        edit[1][1][0:1] = b"V"                      # changed after the call, before the next
+        s.reset(src)
+        s.apply_batch(edit)
+        assert s.result() == ro.apply_batch(src, [(bytes(p), bytes(r)) for p, r in edit])
+
+
 def test_unbounded_expansion_reads_the_size_first(lib):
     # 2-byte patterns to 64-byte replacements over 16 MiB: the worst case (32 x the
     # input) is beyond what a pass may allocate up front, so the exact size is read
